@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused 3D O(8) velocity-stress step on one MI355X.
+
+Workload (one step = one leapfrog time step of the hot path): 300 x 50 x 300 grid, dx = 10 m,
+free surface on top and 20-cell CPML on the other five faces, five seeded horizontal layers
+(vp 1800..4500 m/s, vs = vp/1.73, Gardner density), 15 Hz Ricker vertical force, the default
+150-receiver surface spread sampled every 2 steps into the device gather.  dt is the
+reference's automatic CFL step, 847 us: the requested 1 ms violates the O(8) 3D CFL bound and
+the reference rejects it.
+
+Prints ONE JSON line: Gcell-updates/s over the timed steps (inputs resident in HBM), the HBM
+roofline of the dominant kernel measured with HIP events on the session stream, and the CPU
+oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s 3D O(8) VS-FD + %HBM roofline; batched FWI-grad shots/s vs plain"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--family", type=int, default=1)
+    p.add_argument("--tile-y", type=int, default=0)
+    p.add_argument("--z-chunk", type=int, default=0)
+    p.add_argument("--kernel-steps", type=int, default=200, help="steps timed per kernel with HIP events")
+    p.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--grid", default="300x50x300")
+    return p.parse_args()
+
+
+def cpu_info():
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(ctx, model, src, rec, steps_hint):
+    """FP32 CPU oracle (OpenMP over z) on the same workload, bounded to ~10-30 s."""
+    from oracle import pyoracle as O
+    cores, cpu_model = cpu_info()
+    threads = min(cores, 16)
+    O.set_threads(threads)
+    o = O.Oracle3D(ctx, 32)
+    o.upload_model(*model)
+    o.set_receivers(rec)
+    warm = o.time_steps(src, 0, 1)
+    steps = steps_hint if steps_hint > 0 else max(2, min(60, int(15.0 / max(warm, 1e-3))))
+    secs = o.time_steps(src, 1, steps)
+    o.close()
+    n_upd = (ctx.gx - 8) * (ctx.gy - 8) * (ctx.gz - 8)
+    return {
+        "value": round(n_upd * steps / secs / 1e9, 5),
+        "unit": "Gcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 oracle (line-faithful C++ restatement, OpenMP {threads} threads, -O3 -march=x86-64-v3, "
+                  f"no FP contraction) on the same 300x50x300 workload, steps 1..{steps} after 1 warm-up step, "
+                  f"{secs:.2f} s; host CPU: {cpu_model}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
+        sys.exit(2)
+
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    from cmakepro_cuda_seissim_v2_amd import workloads as W
+
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    ctx = W.config2_context(nx, ny, nz)
+    model = W.layered_model(ctx.nx, ctx.ny, ctx.nz)
+    src = W.config2_source(ctx)
+    rec = W.config2_receivers(ctx)
+    sb = S.step_bytes(ctx)
+
+    g = S.GPUSimData3D(ctx)
+    if args.tile_y or args.z_chunk or args.family != 1:
+        f0, ty0, zc0 = g.get_tiles()
+        g.set_tiles(args.family, args.tile_y or ty0, args.z_chunk or zc0)
+    family, tile_y, z_chunk = g.get_tiles()
+    g.upload_model(*model)
+    g.set_receivers(rec)
+    stride = 2
+    total_steps = args.warmup + args.steps
+    g.gather_begin(max(1, total_steps // stride), stride)
+
+    print(f"bench: warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    g.run(src, 0, args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(src, args.warmup, args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    gather = g.gather_read()
+    print(f"bench: timed {args.steps} steps in {secs:.4f} s; gather {gather.shape}", file=sys.stderr, flush=True)
+
+    # Per-kernel durations with HIP events on the session stream.
+    g.gather_end()
+    kt = g.run_timed(src, total_steps, args.kernel_steps)
+    g.synchronize()
+    vel_us = kt.velocity_ms * 1e3 / kt.launches
+    str_us = kt.stress_ms * 1e3 / kt.launches
+    vel_gbs = sb["velocity"] / (vel_us * 1e-6) / 1e9
+    str_gbs = sb["stress"] / (str_us * 1e-6) / 1e9
+    dominant = ("stress", str_us, sb["stress"], str_gbs) if str_us >= vel_us else ("velocity", vel_us, sb["velocity"], vel_gbs)
+    g.release()
+
+    n_upd = sb["n_upd"]
+    value = n_upd * args.steps / secs / 1e9
+    step_gbs = sb["total"] * args.steps / secs / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 5-layer model, Ricker force source)",
+        "config": {
+            "workload": f"3D O(8) velocity-stress step, {nx}x{ny}x{nz}, dx=10 m, free top + 20-cell CPML, "
+                        f"15 Hz force, 150 receivers stride 2, dt=847 us (1 ms violates CFL)",
+            "grid": [nx, ny, nz], "padded": [ctx.gx, ctx.gy, ctx.gz],
+            "n_upd": n_upd, "n_psi": sb["n_psi"], "dt_us": round(ctx.dt * 1e6),
+            "kernel_family": "stream-2.5D" if family == 1 else "direct",
+            "tile_y": tile_y, "z_chunk": z_chunk,
+            "step_hbm_gbs_algorithmic": round(step_gbs, 1),
+            "step_frac_of_peak": round(step_gbs / HBM_PEAK_GBS, 4),
+            "velocity_kernel_us": round(vel_us, 2), "stress_kernel_us": round(str_us, 2),
+            "velocity_kernel_gbs": round(vel_gbs, 1), "stress_kernel_gbs": round(str_gbs, 1),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"{dominant[0]}_stream" if family == 1 else f"{dominant[0]}_direct",
+            "achieved": round(dominant[3], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dominant[3] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dominant[2],
+            "avg_launch_us": round(dominant[1], 3),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ctx, model, src, rec, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

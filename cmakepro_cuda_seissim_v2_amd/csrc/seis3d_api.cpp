@@ -1,0 +1,1043 @@
+// seis3d_api.cpp - C ABI and host session layer of the 3D solver.
+//
+// Restates the reference's Seis3D host API (cuda/Cuda_Seis3D.cu:433-733) and
+// the cadence of its session driver (src/Seis3D_Sim.cpp:120-153,
+// src/Seis3D/Acquisition3D.cpp:21-37,188-242) on HIP, with one non-blocking
+// stream per session and no per-sample host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/seis3d.h"
+#include "seis3d_kernels.hpp"
+
+namespace {
+
+constexpr int kRadius = 4;
+constexpr std::size_t kMaxPhysicalNodes = std::size_t(300) * 300 * 400;   // Common3D.h:10
+constexpr int kMaxVolumeAxis = 256;                                        // Common3D.h:11
+constexpr int kMaxVolumeOutput = 1536;                                     // Common3D.h:12
+constexpr int kMaxReceivers = 300;                                         // Common3D.h:14
+constexpr int kMaxSegySamples = 32767;                                     // Common3D.h:15
+constexpr int kMaxSegyIntervalUs = 32767;                                  // Common3D.h:16
+constexpr float kSourceNorm = 10.8387785335f;                              // (1 + 2 exp(-1/2))^3
+
+thread_local std::string g_error;
+
+struct Failure {
+    int code;
+    std::string message;
+};
+
+[[noreturn]] void fail(int code, const std::string& message) { throw Failure{code, message}; }
+[[noreturn]] void invalid(const std::string& message) { fail(SEIS3D_EINVAL, message); }
+
+void check(hipError_t e, const char* operation) {
+    if (e != hipSuccess) fail(SEIS3D_EHIP, std::string("Seis3D / ") + operation + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return SEIS3D_OK;
+    } catch (const Failure& f) {
+        g_error = f.message;
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "Seis3D: host allocation failed.";
+        return SEIS3D_EHIP;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SEIS3D_EHIP;
+    }
+}
+
+// validMaterial3D (cuda/Cuda_Seis3D.cu:30-34).
+bool valid_material(float vp, float vs, float rho) {
+    return std::isfinite(vp) && std::isfinite(vs) && std::isfinite(rho) && vp > 0 && vp <= 30000 && vs >= 0 &&
+           vs < vp && vp * vp > (4.0f / 3.0f) * vs * vs && rho > 0 && rho <= 50000;
+}
+
+int active_pml(const seis3d_ctx& c) { return c.reflecting ? 0 : c.pml; }
+
+// validateContext3D (cuda/Cuda_Seis3D.cu:433-483).
+void validate(seis3d_ctx& c) {
+    if (c.reflecting) c.free_surface = 0;
+    if (c.model_kind < 0 || c.model_kind > SEIS3D_MODEL_SALT_DOME || !std::isfinite(c.interface_depth) ||
+        c.interface_depth < .1f || c.interface_depth > .9f || !std::isfinite(c.layer_dip) ||
+        std::abs(c.layer_dip) > .6f || !std::isfinite(c.layer_thickness) || c.layer_thickness < .03f ||
+        c.layer_thickness > .5f || !std::isfinite(c.fault_throw) || std::abs(c.fault_throw) > .4f ||
+        !std::isfinite(c.anomaly_radius) || c.anomaly_radius < .05f || c.anomaly_radius > .4f)
+        invalid("Seis3D: invalid procedural model geometry.");
+    if (c.nx < 16 || c.ny < 16 || c.nz < 16) invalid("Seis3D: every physical dimension must be at least 16.");
+    if (std::size_t(c.nx) > kMaxPhysicalNodes / std::size_t(c.ny) ||
+        std::size_t(c.nx) * c.ny > kMaxPhysicalNodes / std::size_t(c.nz))
+        invalid("Seis3D: physical node count exceeds 300 x 300 x 400 (36,000,000).");
+    if (c.pml < 8 || c.pml > 40 || !std::isfinite(c.spacing) || c.spacing <= 0 || !std::isfinite(c.frequency) ||
+        c.frequency <= 0 || c.frequency > 10000 || !std::isfinite(c.cfl) || c.cfl <= 0 || c.cfl > 0.9f)
+        invalid("Seis3D: invalid CPML, spacing, frequency or CFL safety factor.");
+    if (!valid_material(c.vp, c.vs, c.density) || !valid_material(c.layer_vp, c.layer_vs, c.layer_density))
+        invalid("Seis3D: Vp/Vs/rho must be finite with positive bulk modulus.");
+    if (c.model_kind != SEIS3D_MODEL_IMPORTED)
+        c.max_vp = c.model_kind == SEIS3D_MODEL_UNIFORM ? c.vp : std::max(c.vp, c.layer_vp);
+    if (!std::isfinite(c.max_vp) || c.max_vp <= 0 || c.max_vp > 30000) invalid("Seis3D: invalid model maximum Vp.");
+    const int pml = active_pml(c);
+    c.ox = c.oy = pml + 4;
+    c.oz = c.free_surface ? 4 : pml + 4;
+    c.gx = c.nx + 2 * pml + 8;
+    c.gy = c.ny + 2 * pml + 8;
+    c.gz = c.nz + (c.free_surface ? 1 : 2) * pml + 8;
+    c.cells = std::uint64_t(c.gx) * c.gy * c.gz;
+    constexpr double kSumWeights = 1225.0 / 1024 + 245.0 / 3072 + 49.0 / 5120 + 5.0 / 7168;
+    const double safe = c.cfl * c.spacing / (std::sqrt(3.0) * c.max_vp * kSumWeights);
+    const double us = std::floor(safe * 1.0e6);   // integer-microsecond dt for exact SEG-Y intervals
+    if (us < 1 || us > 65535) invalid("Seis3D: safe dt is outside the implemented 1..65535 microsecond range.");
+    c.safe_dt = float(us * 1.0e-6);
+    c.dt = c.safe_dt;
+    if (c.manual_dt) {
+        const double requested = double(c.requested_dt_ms) * 1000;
+        const double rounded = std::round(requested);
+        if (!std::isfinite(requested) || rounded < 1 || rounded > 65535)
+            invalid("Seis3D: manual dt must be between 0.001 and 65.535 ms.");
+        if (std::abs(requested - rounded) > 0.05)
+            invalid("Seis3D: manual dt must use whole microseconds (0.001 ms increments).");
+        if (rounded > us)
+            invalid("Seis3D: manual dt exceeds the model CFL ceiling; reduce dt or select Automatic CFL.");
+        c.dt = float(rounded * 1.0e-6);
+    }
+}
+
+// slabCount3D (cuda/Cuda_Seis3D.cu:37-42): reference slab cell counts.
+std::size_t slab_count(const seis3d_ctx& c, int axis) {
+    if (c.reflecting) return 0;
+    if (axis == 0) return std::size_t(2 * c.pml) * c.gy * c.gz;
+    if (axis == 1) return std::size_t(2 * c.pml) * c.gx * c.gz;
+    return std::size_t((c.free_surface ? 1 : 2) * c.pml) * c.gx * c.gy;
+}
+
+// makeViewLayout3D (cuda/Cuda_Seis3D.cu:486-501): display domain sizes.
+struct ViewLayout {
+    std::size_t slice_offset[3]{};
+    std::size_t volume_offset = 0, total = 0;
+    int domain[3]{}, origin[3]{};
+    int volume[3]{};
+};
+ViewLayout make_view_layout(const seis3d_ctx& c) {
+    ViewLayout l;
+    const int pml = active_pml(c);
+    const int nx = c.nx + 2 * pml, ny = c.ny + 2 * pml, nz = c.nz + (c.free_surface ? 1 : 2) * pml;
+    l.domain[0] = nx; l.domain[1] = ny; l.domain[2] = nz;
+    l.origin[0] = -pml; l.origin[1] = -pml; l.origin[2] = c.free_surface ? 0 : -pml;
+    l.slice_offset[1] = std::size_t(nx) * nz;
+    l.slice_offset[2] = l.slice_offset[1] + std::size_t(ny) * nz;
+    l.volume_offset = l.slice_offset[2] + std::size_t(nx) * ny;
+    l.volume[0] = std::min(kMaxVolumeAxis, nx);
+    l.volume[1] = std::min(kMaxVolumeAxis, ny);
+    l.volume[2] = std::min(kMaxVolumeAxis, nz);
+    l.total = l.volume_offset + std::size_t(l.volume[0]) * l.volume[1] * l.volume[2];
+    return l;
+}
+
+// makeCoefficients3D (cuda/Cuda_Seis3D.cu:45-64).
+std::vector<float> make_coefficients(const seis3d_ctx& c, int axis) {
+    const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
+    std::vector<float> values(std::size_t(n) * 6);
+    const double sigma_max = -4.0 * c.max_vp * std::log(1.0e-6) / (2.0 * c.pml * c.spacing);
+    for (int half = 0; half < 2; ++half)
+        for (int i = 0; i < n; ++i) {
+            const double coordinate = i + 0.5 * half;
+            double depth = (coordinate - (n - kRadius - c.pml - 0.5)) / c.pml;
+            if (!(axis == 2 && c.free_surface)) depth = std::max(depth, (kRadius + c.pml - 0.5 - coordinate) / c.pml);
+            depth = std::clamp(depth, 0.0, 1.0);
+            const double power = depth * depth * depth;
+            const double sigma = sigma_max * power, kappa = 1.0 + 2.0 * power;
+            const double alpha = depth > 0 ? 3.141592653589793 * c.frequency * (1.0 - depth) : 0.0;
+            const double b = std::exp(-(sigma / kappa + alpha) * c.dt);
+            const double a = sigma > 0 ? sigma * (b - 1.0) / (kappa * (sigma + kappa * alpha)) : 0.0;
+            const std::size_t o = (std::size_t(half) * n + i) * 3;
+            values[o] = float(a);
+            values[o + 1] = float(b);
+            values[o + 2] = float(1.0 / kappa);
+        }
+    return values;
+}
+
+// Ricker sample of sourceKernel3D (cuda/Cuda_Seis3D.cu:319-320): FP64 phase,
+// rounded to float.
+float ricker(float frequency, double time) {
+    const double phase = 3.141592653589793 * frequency * (time - 1.2 / frequency);
+    return float((1 - 2 * phase * phase) * std::exp(-phase * phase));
+}
+
+std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
+
+// Device allocation plan for one context.
+struct Plan {
+    int px = 0;
+    std::size_t plane = 0, cells = 0;   // pitched cells
+    std::size_t slab[3]{};              // floats per CPML slot
+    std::size_t bytes = 0;
+};
+Plan plan_for(const seis3d_ctx& c) {
+    Plan p;
+    p.px = int(align_up(std::size_t(c.gx), 64));
+    p.plane = std::size_t(p.px) * c.gy;
+    p.cells = p.plane * c.gz;
+    const int pml = active_pml(c);
+    if (pml > 0) {
+        p.slab[0] = std::size_t(c.gz) * c.gy * (2 * pml);
+        p.slab[1] = std::size_t(c.gz) * (2 * pml) * p.px;
+        p.slab[2] = std::size_t((c.free_surface ? 1 : 2) * pml) * c.gy * p.px;
+    }
+    p.bytes = p.cells * 12 * sizeof(float);
+    for (int a = 0; a < 3; ++a) p.bytes += p.slab[a] * 6 * sizeof(float);
+    p.bytes += std::size_t(c.gx + c.gy + c.gz) * 6 * sizeof(float);
+    return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Session
+// ---------------------------------------------------------------------------
+
+struct seis3d_session {
+    seis3d_ctx c{};
+    Plan plan;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* fields[9]{};
+    float* inv_rho = nullptr;
+    float* lambda = nullptr;
+    float* mu = nullptr;
+    float* psi[3]{};
+    float* coef[3]{};
+    unsigned int* flags = nullptr;
+    int* receivers = nullptr;          // padded coordinates, 3 ints each
+    float* receiver_values = nullptr;
+    float* host_values = nullptr;      // pinned
+    int receiver_count = 0;
+    float* gather = nullptr;           // [sample][receiver][3]
+    int gather_target = 0, gather_count = 0, gather_stride = 1, gather_receivers = 0;
+    bool recording = false;
+    bool ghosts_valid = true;          // σ ghost planes consistent with the interior
+    seis3d::KernelConfig kernels;
+
+    seis3d::DevStep base() const {
+        seis3d::DevStep P{};
+        P.g.gx = c.gx; P.g.gy = c.gy; P.g.gz = c.gz;
+        P.g.px = plan.px;
+        P.g.plane = static_cast<long long>(plan.plane);
+        P.g.pml = active_pml(c);
+        P.g.free_surface = c.free_surface;
+        P.g.tz = (c.free_surface ? 1 : 2) * P.g.pml;
+        for (int a = 0; a < 3; ++a) P.g.slab[a] = static_cast<long long>(plan.slab[a]);
+        for (int f = 0; f < 9; ++f) P.p.f[f] = fields[f];
+        P.p.inv_rho = inv_rho; P.p.lambda = lambda; P.p.mu = mu;
+        for (int a = 0; a < 3; ++a) { P.p.psi[a] = psi[a]; P.p.coef[a] = coef[a]; }
+        P.p.flags = flags;
+        P.dt = c.dt;
+        P.invh = 1.0f / c.spacing;
+        P.src.active = 0;
+        return P;
+    }
+
+    seis3d::MaterialParams material_params() const {
+        seis3d::MaterialParams m{};
+        m.nx = c.nx; m.ny = c.ny; m.nz = c.nz; m.ox = c.ox; m.oy = c.oy; m.oz = c.oz;
+        m.kind = c.model_kind;
+        m.vp = c.vp; m.vs = c.vs; m.rho = c.density;
+        m.layer_vp = c.layer_vp; m.layer_vs = c.layer_vs; m.layer_rho = c.layer_density;
+        m.interface_depth = c.interface_depth; m.layer_dip = c.layer_dip; m.layer_thickness = c.layer_thickness;
+        m.fault_throw = c.fault_throw; m.anomaly_radius = c.anomaly_radius;
+        return m;
+    }
+
+    void release() noexcept {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& p : fields) { (void)hipFree(p); p = nullptr; }
+        (void)hipFree(inv_rho); (void)hipFree(lambda); (void)hipFree(mu);
+        inv_rho = lambda = mu = nullptr;
+        for (int a = 0; a < 3; ++a) { (void)hipFree(psi[a]); (void)hipFree(coef[a]); psi[a] = coef[a] = nullptr; }
+        (void)hipFree(flags); (void)hipFree(receivers); (void)hipFree(receiver_values); (void)hipFree(gather);
+        flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr;
+        if (host_values) (void)hipHostFree(host_values);
+        host_values = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+void require_session(const seis3d_t* t) {
+    if (!t || !t->stream) fail(SEIS3D_ELIFECYCLE, "Seis3D: simulation not initialized.");
+}
+
+void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what) {
+    if (count == 0) return;
+    check(hipMalloc(&p, count * sizeof(float)), what);
+    check(hipMemsetAsync(p, 0, count * sizeof(float), s), what);
+}
+
+void upload_coefficients(seis3d_t* t, const seis3d_ctx& c, float* const dst[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const auto v = make_coefficients(c, a);
+        check(hipMemcpyAsync(dst[a], v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, t->stream),
+              "upload CPML coefficients");
+    }
+    check(hipStreamSynchronize(t->stream), "finish coefficient upload");
+}
+
+void check_source(const seis3d_ctx& c, const seis3d_source& s) {
+    if (s.x < 1 || s.x >= c.nx - 1 || s.y < 1 || s.y >= c.ny - 1 || s.z < 2 || s.z >= c.nz - 1 ||
+        !std::isfinite(s.amplitude))
+        invalid("Seis3D: source must be inside the physical volume, at depth >=2.");
+    if (s.kind != SEIS3D_EXPLOSION && s.kind != SEIS3D_FORCE) invalid("Seis3D: invalid source kind.");
+    for (float d : s.direction)
+        if (!std::isfinite(d)) invalid("Seis3D: invalid force direction.");
+}
+
+// Source term at time `time` (sourceKernel3D, cuda/Cuda_Seis3D.cu:313-332).
+seis3d::DevSource make_source(const seis3d_ctx& c, const seis3d_source& s, double time) {
+    seis3d::DevSource d{};
+    d.active = 1;
+    d.x = s.x + c.ox; d.y = s.y + c.oy; d.z = s.z + c.oz;
+    d.scale = c.dt * s.amplitude * ricker(c.frequency, time);
+    for (int r = 0; r < 4; ++r) d.w[r] = std::exp(-0.5f * float(r));
+    d.norm = kSourceNorm;
+    for (int v = 0; v < 3; ++v) d.dir[v] = s.direction[v];
+    return d;
+}
+
+void default_tiles(seis3d_t* t) {
+    // Aim for at least two workgroups per CU on a 256-CU part.
+    const int nbx = (t->c.gx - 8 + 63) / 64, nby = (t->c.gy - 8 + 7) / 8;
+    const int uz = t->c.gz - 8;
+    int nbz = std::max(1, (512 + nbx * nby - 1) / (nbx * nby));
+    int chunk = std::max(8, (uz + nbz - 1) / nbz);
+    t->kernels.family = 1;
+    t->kernels.tile_y = 8;
+    t->kernels.z_chunk = chunk;
+}
+
+// One runStep3D (cuda/Cuda_Seis3D.cu:602-624), enqueued on the session stream.
+// Optional events bracket the two fused half-steps.
+void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev = nullptr) {
+    const seis3d_ctx& c = t->c;
+    seis3d::DevStep P = t->base();
+    if (c.free_surface && !t->ghosts_valid) check(seis3d::launch_stress_ghost(P, t->stream), "kernel launch");
+    if (c.reflecting) check(seis3d::launch_reflecting_walls(P, false, t->stream), "kernel launch");
+    seis3d::DevStep V = P;
+    if (s.kind == SEIS3D_FORCE) V.src = make_source(c, s, double(n) * c.dt);
+    if (ev) check(hipEventRecord(ev[0], t->stream), "record event");
+    check(seis3d::launch_velocity(V, t->kernels, t->stream), "kernel launch");
+    if (ev) check(hipEventRecord(ev[1], t->stream), "record event");
+    if (c.free_surface) check(seis3d::launch_velocity_ghost(P, c.spacing, t->stream), "kernel launch");
+    if (c.reflecting) check(seis3d::launch_reflecting_walls(P, true, t->stream), "kernel launch");
+    seis3d::DevStep S = P;
+    if (s.kind == SEIS3D_EXPLOSION) S.src = make_source(c, s, (double(n) + 0.5) * c.dt);
+    if (ev) check(hipEventRecord(ev[2], t->stream), "record event");
+    check(seis3d::launch_stress(S, t->kernels, t->stream), "kernel launch");
+    if (ev) check(hipEventRecord(ev[3], t->stream), "record event");
+    if (c.free_surface) check(seis3d::launch_stress_ghost(P, t->stream), "kernel launch");
+    if (c.reflecting) check(seis3d::launch_reflecting_walls(P, false, t->stream), "kernel launch");
+    t->ghosts_valid = true;
+}
+
+// Sampling cadence of advanceSimulation3D (src/Seis3D_Sim.cpp:127-133).
+void maybe_sample(seis3d_t* t, std::uint64_t n) {
+    if (!t->recording || t->gather_count >= t->gather_target) return;
+    if ((n + 1) % std::uint64_t(t->gather_stride) != 0) return;
+    float* dst = t->gather + std::size_t(t->gather_count) * t->gather_receivers * 3;
+    check(seis3d::launch_receivers(t->base(), t->receivers, t->receiver_count, dst, t->stream), "kernel launch");
+    ++t->gather_count;
+    if (t->gather_count >= t->gather_target) t->recording = false;
+}
+
+void read_flag(seis3d_t* t, unsigned int* flag) {
+    check(hipMemcpyAsync(flag, t->flags, sizeof(unsigned int), hipMemcpyDeviceToHost, t->stream),
+          "check capture stability");
+}
+
+}  // namespace
+
+extern "C" {
+
+void seis3d_default_ctx(seis3d_ctx* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->nx = 300; c->ny = 50; c->nz = 300;
+    c->pml = 12;
+    c->free_surface = 1;
+    c->reflecting = 0;
+    c->spacing = 10.0f; c->cfl = 0.85f; c->dt = 0.0f;
+    c->manual_dt = 0;
+    c->requested_dt_ms = 0.5f;
+    c->safe_dt = 0.0f;
+    c->vp = 3000.0f; c->vs = 1700.0f; c->density = 2200.0f;
+    c->layer_vp = 4200.0f; c->layer_vs = 2400.0f; c->layer_density = 2500.0f;
+    c->max_vp = 4200.0f; c->frequency = 20.0f;
+    c->model_kind = SEIS3D_MODEL_TWO_LAYER;
+    c->interface_depth = 0.5f; c->layer_dip = 0.25f; c->layer_thickness = 0.15f;
+    c->fault_throw = 0.18f; c->anomaly_radius = 0.187f;
+}
+
+void seis3d_default_source(seis3d_source* s) {
+    if (!s) return;
+    s->x = 48; s->y = 48; s->z = 24;
+    s->kind = SEIS3D_EXPLOSION;
+    s->amplitude = 1.0e8f;
+    s->direction[0] = 0.0f; s->direction[1] = 0.0f; s->direction[2] = 1.0f;
+}
+
+void seis3d_default_survey(seis3d_survey* q) {
+    if (!q) return;
+    q->receiver_count_x = 50; q->receiver_count_y = 3;
+    q->receiver_origin[0] = 27; q->receiver_origin[1] = 1; q->receiver_origin[2] = 0;
+    q->receiver_spacing[0] = 5; q->receiver_spacing[1] = 23;
+    q->shot_count_x = 3; q->shot_count_y = 2;
+    q->shot_origin[0] = 24; q->shot_origin[1] = 24; q->shot_origin[2] = 12;
+    q->shot_spacing[0] = 20; q->shot_spacing[1] = 20;
+}
+
+int seis3d_validate_ctx(seis3d_ctx* c) {
+    return guarded([&] {
+        if (!c) invalid("Seis3D: null context.");
+        validate(*c);
+    });
+}
+
+// estimateMemory3D (cuda/Cuda_Seis3D.cu:504-516): the reference's budget,
+// including its GL staging, for callers that compare budgets.
+int seis3d_estimate_memory(const seis3d_ctx* ctx, seis3d_budget* b) {
+    return guarded([&] {
+        if (!ctx || !b) invalid("Seis3D: null argument.");
+        seis3d_ctx c = *ctx;
+        validate(c);
+        b->field_bytes = c.cells * 12 * sizeof(float);
+        b->cpml_bytes = 0;
+        for (int a = 0; a < 3; ++a) b->cpml_bytes += slab_count(c, a) * 6 * sizeof(float);
+        const auto l = make_view_layout(c);
+        b->view_bytes = l.total * 4 * sizeof(float) + l.volume_offset * 4 +
+                        std::size_t(kMaxVolumeOutput) * kMaxVolumeOutput * 4;
+        b->total_bytes = b->field_bytes + b->cpml_bytes + b->view_bytes +
+                         std::size_t(c.gx + c.gy + c.gz) * 6 * sizeof(float) +
+                         kMaxReceivers * (sizeof(seis3d_receiver) + 3 * sizeof(float)) + 2 * 1024 * 1024;
+    });
+}
+
+int seis3d_device_bytes(const seis3d_ctx* ctx, uint64_t* out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("Seis3D: null argument.");
+        seis3d_ctx c = *ctx;
+        validate(c);
+        *out = plan_for(c).bytes;
+    });
+}
+
+int seis3d_cpml_coefficients(const seis3d_ctx* ctx, int axis, float* out, size_t capacity) {
+    return guarded([&] {
+        if (!ctx || !out || axis < 0 || axis > 2) invalid("Seis3D: invalid coefficient request.");
+        const auto v = make_coefficients(*ctx, axis);
+        if (capacity < v.size()) invalid("Seis3D: coefficient buffer too small.");
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+// fitSurvey3D (src/Seis3D_Sim.cpp:42-60).
+int seis3d_fit_survey(const seis3d_ctx* ctx, seis3d_survey* s) {
+    return guarded([&] {
+        if (!ctx || !s) invalid("Seis3D: null argument.");
+        const seis3d_ctx& c = *ctx;
+        s->receiver_count_x = std::clamp(s->receiver_count_x, 1, std::min(kMaxReceivers, c.nx - 2));
+        s->receiver_count_y = std::clamp(s->receiver_count_y, 1, std::min(kMaxReceivers / s->receiver_count_x, c.ny - 2));
+        s->shot_count_x = std::clamp(s->shot_count_x, 1, 100);
+        s->shot_count_y = std::clamp(s->shot_count_y, 1, 100);
+        s->receiver_origin[2] = 0;
+        s->receiver_spacing[0] = std::max(1, (c.nx - 3) / std::max(1, s->receiver_count_x - 1));
+        s->receiver_spacing[1] = std::max(1, (c.ny - 3) / std::max(1, s->receiver_count_y - 1));
+        s->receiver_origin[0] = s->receiver_count_x == 1 ? c.nx / 2
+                                : (c.nx - 1 - (s->receiver_count_x - 1) * s->receiver_spacing[0]) / 2;
+        s->receiver_origin[1] = s->receiver_count_y == 1 ? c.ny / 2
+                                : (c.ny - 1 - (s->receiver_count_y - 1) * s->receiver_spacing[1]) / 2;
+        s->shot_origin[0] = c.nx / 4;
+        s->shot_origin[1] = c.ny / 4;
+        s->shot_origin[2] = std::max(2, c.nz / 8);
+        s->shot_spacing[0] = std::max(1, (c.nx - 2 - s->shot_origin[0]) / std::max(1, s->shot_count_x - 1));
+        s->shot_spacing[1] = std::max(1, (c.ny - 2 - s->shot_origin[1]) / std::max(1, s->shot_count_y - 1));
+    });
+}
+
+// configureReceivers3D (src/Seis3D/Acquisition3D.cpp:21-37).
+int seis3d_survey_receivers(const seis3d_ctx* ctx, const seis3d_survey* q, seis3d_receiver* out, int* count) {
+    return guarded([&] {
+        if (!ctx || !q || !count) invalid("Seis3D: null argument.");
+        const seis3d_ctx& c = *ctx;
+        if (q->receiver_count_x < 1 || q->receiver_count_y < 1 || q->receiver_count_x > kMaxReceivers ||
+            q->receiver_count_y > kMaxReceivers ||
+            std::int64_t(q->receiver_count_x) * q->receiver_count_y > kMaxReceivers || q->receiver_spacing[0] < 1 ||
+            q->receiver_spacing[1] < 1)
+            fail(SEIS3D_EHIP, "Receiver mesh must have 1..100 total channels (X times Y) and positive spacing.");
+        const auto last_x = std::int64_t(q->receiver_origin[0]) + std::int64_t(q->receiver_count_x - 1) * q->receiver_spacing[0];
+        const auto last_y = std::int64_t(q->receiver_origin[1]) + std::int64_t(q->receiver_count_y - 1) * q->receiver_spacing[1];
+        if (q->receiver_origin[0] < 0 || q->receiver_origin[1] < 0 || last_x >= c.nx || last_y >= c.ny ||
+            q->receiver_origin[2] < 0 || q->receiver_origin[2] >= c.nz)
+            fail(SEIS3D_EHIP, "Receiver mesh exceeds the model.");
+        const int n = q->receiver_count_x * q->receiver_count_y;
+        if (!out || *count < n) { *count = n; invalid("Seis3D: receiver buffer too small."); }
+        int k = 0;
+        for (int y = 0; y < q->receiver_count_y; ++y)
+            for (int x = 0; x < q->receiver_count_x; ++x)
+                out[k++] = {q->receiver_origin[0] + x * q->receiver_spacing[0],
+                            q->receiver_origin[1] + y * q->receiver_spacing[1], q->receiver_origin[2]};
+        *count = n;
+    });
+}
+
+// positionSequenceShot3D (src/Seis3D/Acquisition3D.cpp:233-242); src keeps
+// its kind, amplitude and direction.
+int seis3d_sequence_shot(const seis3d_ctx* ctx, const seis3d_survey* q, int index, seis3d_source* src) {
+    return guarded([&] {
+        if (!ctx || !q || !src) invalid("Seis3D: null argument.");
+        const seis3d_ctx& c = *ctx;
+        if (q->shot_count_x < 1 || q->shot_count_y < 1 || q->shot_count_x > 100 || q->shot_count_y > 100 ||
+            q->shot_spacing[0] < 1 || q->shot_spacing[1] < 1)
+            fail(SEIS3D_EHIP, "Invalid 3D source spread.");
+        const auto last_x = std::int64_t(q->shot_origin[0]) + std::int64_t(q->shot_count_x - 1) * q->shot_spacing[0];
+        const auto last_y = std::int64_t(q->shot_origin[1]) + std::int64_t(q->shot_count_y - 1) * q->shot_spacing[1];
+        if (q->shot_origin[0] < 1 || q->shot_origin[1] < 1 || last_x >= c.nx - 1 || last_y >= c.ny - 1 ||
+            q->shot_origin[2] < 2 || q->shot_origin[2] >= c.nz - 1)
+            fail(SEIS3D_EHIP, "Source spread exceeds the 3D model.");
+        if (index < 0 || index >= q->shot_count_x * q->shot_count_y) invalid("Seis3D: shot index out of range.");
+        src->x = q->shot_origin[0] + (index % q->shot_count_x) * q->shot_spacing[0];
+        src->y = q->shot_origin[1] + (index / q->shot_count_x) * q->shot_spacing[1];
+        src->z = q->shot_origin[2];
+    });
+}
+
+// startGather3D timing (src/Seis3D/Acquisition3D.cpp:201-209).
+int seis3d_gather_timing(const seis3d_ctx* c, int stride, int* interval_us, double* first_sample_time) {
+    return guarded([&] {
+        if (!c || stride < 1 || stride > 10000) invalid("Seis3D: stride must be 1..10000.");
+        const auto interval = std::llround(double(c->dt) * 1e6) * stride;
+        if (interval > kMaxSegyIntervalUs) fail(SEIS3D_EHIP, "SEG-Y rev1 sample interval must not exceed 32767 us.");
+        if (interval_us) *interval_us = int(interval);
+        if (first_sample_time) *first_sample_time = (stride - 0.5) * c->dt;
+    });
+}
+
+float seis3d_ricker(float frequency, double time) { return ricker(frequency, time); }
+
+// initializeGpu3D (cuda/Cuda_Seis3D.cu:519-555).
+int seis3d_create(const seis3d_ctx* ctx, seis3d_t** out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("Seis3D: null argument.");
+        if (*out) fail(SEIS3D_ELIFECYCLE, "Seis3D: release the previous session before initializing.");
+        seis3d_ctx c = *ctx;
+        validate(c);
+        if (c.gx != ctx->gx || c.gy != ctx->gy || c.gz != ctx->gz || c.dt != ctx->dt)
+            invalid("Seis3D: context must pass seis3d_validate_ctx before initialization.");
+        auto* t = new seis3d_session();
+        t->c = c;
+        t->plan = plan_for(c);
+        try {
+            check(hipGetDevice(&t->device), "query current device");
+            std::size_t free_bytes = 0, total_bytes = 0;
+            check(hipMemGetInfo(&free_bytes, &total_bytes), "query device memory");
+            const std::size_t reserve = std::max(std::size_t(512) * 1024 * 1024, total_bytes / 20);
+            if (free_bytes < reserve || t->plan.bytes > free_bytes - reserve)
+                fail(SEIS3D_EHIP, "Seis3D: device memory budget exceeded; reduce grid size or CPML thickness.");
+            check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "create stream");
+            for (auto& f : t->fields) alloc_floats(f, t->plan.cells, t->stream, "allocate field");
+            alloc_floats(t->inv_rho, t->plan.cells, t->stream, "allocate field");
+            alloc_floats(t->lambda, t->plan.cells, t->stream, "allocate field");
+            alloc_floats(t->mu, t->plan.cells, t->stream, "allocate field");
+            if (!c.reflecting) {
+                for (int a = 0; a < 3; ++a) {
+                    alloc_floats(t->psi[a], t->plan.slab[a] * 6, t->stream, "allocate CPML");
+                    const int n = a == 0 ? c.gx : a == 1 ? c.gy : c.gz;
+                    alloc_floats(t->coef[a], std::size_t(n) * 6, t->stream, "allocate CPML coefficients");
+                }
+                upload_coefficients(t, c, t->coef);
+            }
+            check(hipMalloc(&t->flags, 4 * sizeof(unsigned int)), "allocate statistics");
+            check(hipMemsetAsync(t->flags, 0, 4 * sizeof(unsigned int), t->stream), "clear statistics");
+            check(seis3d::launch_generate_materials(t->base(), t->material_params(), t->stream), "kernel launch");
+            check(hipStreamSynchronize(t->stream), "initialize model");
+            default_tiles(t);
+        } catch (...) {
+            t->release();
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+void seis3d_destroy(seis3d_t* t) {
+    if (!t) return;
+    t->release();
+    delete t;
+}
+
+int seis3d_get_ctx(const seis3d_t* t, seis3d_ctx* out) {
+    return guarded([&] {
+        if (!t || !out) invalid("Seis3D: null argument.");
+        *out = t->c;
+    });
+}
+
+// resetGpu3D (cuda/Cuda_Seis3D.cu:572-577).
+int seis3d_reset(seis3d_t* t) {
+    return guarded([&] {
+        require_session(t);
+        for (auto p : t->fields) check(hipMemsetAsync(p, 0, t->plan.cells * sizeof(float), t->stream), "reset wavefield");
+        for (int a = 0; a < 3; ++a)
+            if (t->psi[a]) check(hipMemsetAsync(t->psi[a], 0, t->plan.slab[a] * 6 * sizeof(float), t->stream), "reset CPML");
+        check(hipMemsetAsync(t->flags, 0, 4 * sizeof(unsigned int), t->stream), "reset diagnostics");
+        t->ghosts_valid = true;
+    });
+}
+
+// refreshCpmlCoefficients3D (cuda/Cuda_Seis3D.cu:580-599): new tables are
+// built and uploaded into fresh buffers before the old ones are swapped out.
+int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx) {
+    return guarded([&] {
+        require_session(t);
+        if (!ctx) invalid("Seis3D: null context.");
+        seis3d_ctx c = *ctx;
+        validate(c);
+        if (c.gx != t->c.gx || c.gy != t->c.gy || c.gz != t->c.gz || c.reflecting != t->c.reflecting ||
+            c.free_surface != t->c.free_surface || c.pml != t->c.pml || c.dt != t->c.dt)
+            invalid("Seis3D: a CPML refresh may only change the source frequency.");
+        if (c.reflecting) { t->c = c; return; }
+        float* replacement[3]{};
+        try {
+            for (int a = 0; a < 3; ++a) {
+                const int n = a == 0 ? c.gx : a == 1 ? c.gy : c.gz;
+                alloc_floats(replacement[a], std::size_t(n) * 6, t->stream, "refresh CPML coefficients");
+            }
+            upload_coefficients(t, c, replacement);
+        } catch (...) {
+            (void)hipStreamSynchronize(t->stream);
+            for (auto p : replacement) (void)hipFree(p);
+            throw;
+        }
+        for (int a = 0; a < 3; ++a) {
+            std::swap(t->coef[a], replacement[a]);
+            (void)hipFree(replacement[a]);
+        }
+        t->c = c;
+    });
+}
+
+int seis3d_read_cpml(seis3d_t* t, int axis, float* out, size_t capacity) {
+    return guarded([&] {
+        require_session(t);
+        if (axis < 0 || axis > 2 || !out) invalid("Seis3D: invalid coefficient request.");
+        if (!t->coef[axis]) invalid("Seis3D: no CPML in reflecting mode.");
+        const int n = axis == 0 ? t->c.gx : axis == 1 ? t->c.gy : t->c.gz;
+        if (capacity < std::size_t(n) * 6) invalid("Seis3D: coefficient buffer too small.");
+        check(hipMemcpyAsync(out, t->coef[axis], std::size_t(n) * 6 * sizeof(float), hipMemcpyDeviceToHost, t->stream),
+              "read CPML coefficients");
+        check(hipStreamSynchronize(t->stream), "read CPML coefficients");
+    });
+}
+
+// uploadModelPlane3D (cuda/Cuda_Seis3D.cu:696-711).
+int seis3d_upload_plane(seis3d_t* t, int z, const float* values) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        if (z < 0 || z >= c.nz || !values) invalid("Seis3D: invalid model plane.");
+        const std::size_t n = std::size_t(c.nx) * c.ny;
+        std::vector<float> material(n * 3);
+        for (std::size_t i = 0; i < n; ++i) {
+            const float vp = values[3 * i], vs = values[3 * i + 1], rho = values[3 * i + 2];
+            if (!valid_material(vp, vs, rho) || vp > c.max_vp * (1 + 1.0e-6f))
+                invalid("Seis3D: invalid or changed imported material.");
+            material[i] = 1 / rho;
+            material[n + i] = rho * (vp * vp - 2 * vs * vs);
+            material[2 * n + i] = rho * vs * vs;
+        }
+        const std::size_t offset = (std::size_t(z + c.oz) * c.gy + c.oy) * t->plan.px + c.ox;
+        float* dst[3] = {t->inv_rho, t->lambda, t->mu};
+        for (int a = 0; a < 3; ++a)
+            check(hipMemcpy2DAsync(dst[a] + offset, std::size_t(t->plan.px) * 4, material.data() + a * n,
+                                   std::size_t(c.nx) * 4, std::size_t(c.nx) * 4, c.ny, hipMemcpyHostToDevice, t->stream),
+                  "upload material plane");
+        check(hipStreamSynchronize(t->stream), "finish material plane");
+    });
+}
+
+// finishModelUpload3D (cuda/Cuda_Seis3D.cu:714-717).
+int seis3d_finish_upload(seis3d_t* t) {
+    return guarded([&] {
+        require_session(t);
+        check(seis3d::launch_extend_materials(t->base(), t->material_params(), t->stream), "kernel launch");
+        check(hipStreamSynchronize(t->stream), "finish model import");
+    });
+}
+
+// downloadModelPlane3D (cuda/Cuda_Seis3D.cu:720-733).
+int seis3d_download_plane(seis3d_t* t, int z, float* values) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        if (z < 0 || z >= c.nz || !values) invalid("Seis3D: invalid export plane.");
+        const std::size_t n = std::size_t(c.nx) * c.ny;
+        std::vector<float> material(n * 3);
+        const std::size_t offset = (std::size_t(z + c.oz) * c.gy + c.oy) * t->plan.px + c.ox;
+        const float* src[3] = {t->inv_rho, t->lambda, t->mu};
+        for (int a = 0; a < 3; ++a)
+            check(hipMemcpy2DAsync(material.data() + a * n, std::size_t(c.nx) * 4, src[a] + offset,
+                                   std::size_t(t->plan.px) * 4, std::size_t(c.nx) * 4, c.ny, hipMemcpyDeviceToHost,
+                                   t->stream),
+                  "download material plane");
+        check(hipStreamSynchronize(t->stream), "finish model plane export");
+        for (std::size_t i = 0; i < n; ++i) {
+            values[3 * i] = std::sqrt((material[n + i] + 2 * material[2 * n + i]) * material[i]);
+            values[3 * i + 1] = std::sqrt(material[2 * n + i] * material[i]);
+            values[3 * i + 2] = 1 / material[i];
+        }
+    });
+}
+
+// probeMaterial3D (cuda/Cuda_Seis3D.cu:684-693).
+int seis3d_probe(seis3d_t* t, const seis3d_receiver* p, float values[3]) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        if (!p || !values) invalid("Seis3D: null argument.");
+        if (p->x < 0 || p->x >= c.nx || p->y < 0 || p->y >= c.ny || p->z < 0 || p->z >= c.nz) return;
+        const std::size_t i = (std::size_t(p->z + c.oz) * c.gy + (p->y + c.oy)) * t->plan.px + (p->x + c.ox);
+        float inverse = 0, mu = 0, lambda = 0;
+        check(hipMemcpyAsync(&inverse, t->inv_rho + i, 4, hipMemcpyDeviceToHost, t->stream), "probe density");
+        check(hipMemcpyAsync(&mu, t->mu + i, 4, hipMemcpyDeviceToHost, t->stream), "probe mu");
+        check(hipMemcpyAsync(&lambda, t->lambda + i, 4, hipMemcpyDeviceToHost, t->stream), "probe lambda");
+        check(hipStreamSynchronize(t->stream), "finish material probe");
+        values[0] = std::sqrt((lambda + 2 * mu) * inverse);
+        values[1] = std::sqrt(mu * inverse);
+        values[2] = 1 / inverse;
+    });
+}
+
+int seis3d_step(seis3d_t* t, const seis3d_source* src, uint64_t step) {
+    return guarded([&] {
+        require_session(t);
+        if (!src) invalid("Seis3D: null source.");
+        check_source(t->c, *src);
+        enqueue_step(t, *src, step);
+    });
+}
+
+int seis3d_run(seis3d_t* t, const seis3d_source* src, uint64_t first_step, int64_t n_steps) {
+    return guarded([&] {
+        require_session(t);
+        if (!src || n_steps < 0) invalid("Seis3D: invalid run request.");
+        check_source(t->c, *src);
+        for (int64_t k = 0; k < n_steps; ++k) {
+            const std::uint64_t n = first_step + std::uint64_t(k);
+            enqueue_step(t, *src, n);
+            maybe_sample(t, n);
+        }
+    });
+}
+
+int seis3d_synchronize(seis3d_t* t) {
+    return guarded([&] {
+        require_session(t);
+        unsigned int flag = 0;
+        read_flag(t, &flag);
+        check(hipStreamSynchronize(t->stream), "synchronize");
+        if (flag) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; session stopped.");
+    });
+}
+
+// setReceivers3D (cuda/Cuda_Seis3D.cu:649-668): validated, all-or-nothing.
+int seis3d_set_receivers(seis3d_t* t, const seis3d_receiver* r, int count) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        if (count < 0 || count > kMaxReceivers || (count && !r)) invalid("Seis3D: invalid receiver count.");
+        if (t->recording) fail(SEIS3D_ELIFECYCLE, "Seis3D: stop the gather before replacing receivers.");
+        std::vector<int> padded(std::size_t(count) * 3);
+        for (int a = 0; a < count; ++a) {
+            const auto p = r[a];
+            if (p.x < 0 || p.x >= c.nx || p.y < 0 || p.y >= c.ny || p.z < 0 || p.z >= c.nz)
+                invalid("Seis3D: receiver outside the physical domain.");
+            padded[3 * a] = p.x + c.ox;
+            padded[3 * a + 1] = p.y + c.oy;
+            padded[3 * a + 2] = p.z + c.oz;
+        }
+        int* positions = nullptr;
+        float* values = nullptr;
+        float* host = nullptr;
+        try {
+            if (count) {
+                check(hipMalloc(&positions, padded.size() * sizeof(int)), "allocate receivers");
+                check(hipMalloc(&values, std::size_t(count) * 3 * sizeof(float)), "allocate receiver values");
+                check(hipHostMalloc(&host, std::size_t(count) * 3 * sizeof(float), hipHostMallocDefault),
+                      "allocate pinned receivers");
+                check(hipMemcpyAsync(positions, padded.data(), padded.size() * sizeof(int), hipMemcpyHostToDevice,
+                                     t->stream),
+                      "upload receivers");
+            }
+            check(hipStreamSynchronize(t->stream), "finish receiver update");
+        } catch (...) {
+            (void)hipFree(positions);
+            (void)hipFree(values);
+            if (host) (void)hipHostFree(host);
+            throw;
+        }
+        (void)hipFree(t->receivers);
+        (void)hipFree(t->receiver_values);
+        if (t->host_values) (void)hipHostFree(t->host_values);
+        t->receivers = positions;
+        t->receiver_values = values;
+        t->host_values = host;
+        t->receiver_count = count;
+    });
+}
+
+int seis3d_receiver_count(const seis3d_t* t) { return t ? t->receiver_count : 0; }
+
+// sampleReceivers3D (cuda/Cuda_Seis3D.cu:671-681).
+int seis3d_sample_receivers(seis3d_t* t, const float** out) {
+    return guarded([&] {
+        require_session(t);
+        if (!out) invalid("Seis3D: null argument.");
+        *out = nullptr;
+        if (!t->receiver_count) return;
+        check(seis3d::launch_receivers(t->base(), t->receivers, t->receiver_count, t->receiver_values, t->stream),
+              "kernel launch");
+        check(hipMemcpyAsync(t->host_values, t->receiver_values, std::size_t(t->receiver_count) * 3 * sizeof(float),
+                             hipMemcpyDeviceToHost, t->stream),
+              "gather receivers");
+        unsigned int invalid_flag = 0;
+        read_flag(t, &invalid_flag);
+        check(hipStreamSynchronize(t->stream), "finish receiver gather");
+        if (invalid_flag) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; capture stopped before export.");
+        *out = t->host_values;
+    });
+}
+
+// Recording state of startGather3D (src/Seis3D/Acquisition3D.cpp:188-230).
+int seis3d_gather_begin(seis3d_t* t, int target_samples, int stride) {
+    return guarded([&] {
+        require_session(t);
+        if (target_samples < 1 || target_samples > kMaxSegySamples || stride < 1 || stride > 10000)
+            invalid("Gather sample count must be 1..32767 with stride 1..10000.");
+        if (!t->receiver_count) invalid("Seis3D: set receivers before starting a gather.");
+        const std::size_t values = std::size_t(target_samples) * t->receiver_count * 3;
+        float* g = nullptr;
+        check(hipMalloc(&g, values * sizeof(float)), "allocate gather");
+        (void)hipFree(t->gather);
+        t->gather = g;
+        t->gather_target = target_samples;
+        t->gather_stride = stride;
+        t->gather_receivers = t->receiver_count;
+        t->gather_count = 0;
+        t->recording = true;
+    });
+}
+
+int seis3d_gather_count(const seis3d_t* t) { return t ? t->gather_count : 0; }
+
+int seis3d_gather_read(seis3d_t* t, float* out, size_t capacity, int* samples) {
+    return guarded([&] {
+        require_session(t);
+        if (!t->gather) invalid("Seis3D: no gather recorded.");
+        const std::size_t values = std::size_t(t->gather_count) * t->gather_receivers * 3;
+        if (values && (!out || capacity < values)) invalid("Seis3D: gather buffer too small.");
+        unsigned int flag = 0;
+        read_flag(t, &flag);
+        if (values)
+            check(hipMemcpyAsync(out, t->gather, values * sizeof(float), hipMemcpyDeviceToHost, t->stream), "read gather");
+        check(hipStreamSynchronize(t->stream), "finish gather read");
+        if (flag) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; capture stopped before export.");
+        if (samples) *samples = t->gather_count;
+    });
+}
+
+int seis3d_gather_end(seis3d_t* t) {
+    return guarded([&] {
+        require_session(t);
+        check(hipStreamSynchronize(t->stream), "finish gather");
+        (void)hipFree(t->gather);
+        t->gather = nullptr;
+        t->recording = false;
+        t->gather_count = t->gather_target = 0;
+    });
+}
+
+int seis3d_download_field(seis3d_t* t, int field, float* out) {
+    return guarded([&] {
+        require_session(t);
+        if (field < 0 || field > 11 || !out) invalid("Seis3D: invalid field request.");
+        const float* src = field < 9 ? t->fields[field] : field == 9 ? t->inv_rho : field == 10 ? t->lambda : t->mu;
+        const seis3d_ctx& c = t->c;
+        check(hipMemcpy2DAsync(out, std::size_t(c.gx) * 4, src, std::size_t(t->plan.px) * 4, std::size_t(c.gx) * 4,
+                               std::size_t(c.gy) * c.gz, hipMemcpyDeviceToHost, t->stream),
+              "download field");
+        check(hipStreamSynchronize(t->stream), "finish field download");
+    });
+}
+
+int seis3d_upload_field(seis3d_t* t, int field, const float* in) {
+    return guarded([&] {
+        require_session(t);
+        if (field < 0 || field > 8 || !in) invalid("Seis3D: invalid field request.");
+        const seis3d_ctx& c = t->c;
+        check(hipMemcpy2DAsync(t->fields[field], std::size_t(t->plan.px) * 4, in, std::size_t(c.gx) * 4,
+                               std::size_t(c.gx) * 4, std::size_t(c.gy) * c.gz, hipMemcpyHostToDevice, t->stream),
+              "upload field");
+        check(hipStreamSynchronize(t->stream), "finish field upload");
+        t->ghosts_valid = false;
+    });
+}
+
+int seis3d_extract_slice(seis3d_t* t, int axis, int at, int component, float* out, size_t capacity) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        const int n[3] = {c.nx, c.ny, c.nz};
+        if (axis < 0 || axis > 2 || component < 0 || component > 7 || !out || at < 0 || at >= n[axis])
+            invalid("Seis3D: invalid view parameters.");
+        const auto l = make_view_layout(c);
+        seis3d::SliceParams sp{};
+        sp.fixed_axis = axis;
+        sp.fixed = at;
+        sp.u_axis = axis == 0 ? 1 : 0;
+        sp.v_axis = axis == 2 ? 1 : 2;
+        sp.u_origin = l.origin[sp.u_axis];
+        sp.v_origin = l.origin[sp.v_axis];
+        sp.width = l.domain[sp.u_axis];
+        sp.height = l.domain[sp.v_axis];
+        sp.ox = c.ox; sp.oy = c.oy; sp.oz = c.oz;
+        sp.component = component;
+        const std::size_t count = std::size_t(sp.width) * sp.height;
+        if (capacity < count) invalid("Seis3D: slice buffer too small.");
+        float* d = nullptr;
+        check(hipMalloc(&d, count * sizeof(float)), "allocate slice");
+        try {
+            check(seis3d::launch_extract_slice(t->base(), sp, d, t->stream), "kernel launch");
+            check(hipMemcpyAsync(out, d, count * sizeof(float), hipMemcpyDeviceToHost, t->stream), "read slice");
+            unsigned int flag = 0;
+            read_flag(t, &flag);
+            check(hipStreamSynchronize(t->stream), "finish view extraction");
+            (void)hipFree(d);
+            d = nullptr;
+            if (flag) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; session stopped.");
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+    });
+}
+
+int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk) {
+    return guarded([&] {
+        require_session(t);
+        if (family < 0 || family > 1) invalid("Seis3D: kernel family must be 0 (direct) or 1 (stream).");
+        if (family == 1 && tile_y != 4 && tile_y != 8 && tile_y != 16) invalid("Seis3D: tile_y must be 4, 8 or 16.");
+        if (family == 1 && (z_chunk < 1 || z_chunk > 4096)) invalid("Seis3D: z_chunk must be 1..4096.");
+        t->kernels.family = family;
+        if (family == 1) {
+            t->kernels.tile_y = tile_y;
+            t->kernels.z_chunk = z_chunk;
+        }
+    });
+}
+
+int seis3d_get_tiles(const seis3d_t* t, int* family, int* tile_y, int* z_chunk) {
+    return guarded([&] {
+        if (!t) invalid("Seis3D: null session.");
+        if (family) *family = t->kernels.family;
+        if (tile_y) *tile_y = t->kernels.tile_y;
+        if (z_chunk) *z_chunk = t->kernels.z_chunk;
+    });
+}
+
+int seis3d_run_timed(seis3d_t* t, const seis3d_source* src, uint64_t first_step, int64_t n_steps, seis3d_timing* out) {
+    return guarded([&] {
+        require_session(t);
+        if (!src || n_steps < 1 || n_steps > 100000 || !out) invalid("Seis3D: invalid timed run request.");
+        check_source(t->c, *src);
+        std::vector<hipEvent_t> ev(std::size_t(n_steps) * 4 + 2, nullptr);
+        try {
+            for (auto& e : ev) check(hipEventCreate(&e), "create event");
+            check(hipEventRecord(ev[0], t->stream), "record event");
+            for (int64_t k = 0; k < n_steps; ++k) {
+                const std::uint64_t n = first_step + std::uint64_t(k);
+                enqueue_step(t, *src, n, ev.data() + 2 + 4 * k);
+                maybe_sample(t, n);
+            }
+            check(hipEventRecord(ev[1], t->stream), "record event");
+            check(hipEventSynchronize(ev[1]), "finish timed run");
+            double v = 0, s = 0;
+            float ms = 0;
+            for (int64_t k = 0; k < n_steps; ++k) {
+                hipEvent_t* e = ev.data() + 2 + 4 * k;
+                check(hipEventElapsedTime(&ms, e[0], e[1]), "read event");
+                v += ms;
+                check(hipEventElapsedTime(&ms, e[2], e[3]), "read event");
+                s += ms;
+            }
+            check(hipEventElapsedTime(&ms, ev[0], ev[1]), "read event");
+            out->velocity_ms = v;
+            out->stress_ms = s;
+            out->total_ms = ms;
+            out->launches = n_steps;
+        } catch (...) {
+            for (auto e : ev) if (e) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+    });
+}
+
+// Algorithmic bytes of one step: 30 floats per updated cell (velocity: 6
+// stresses + 3 velocities + 1/rho read, 3 velocities written; stress: 3
+// velocities + 6 stresses + lambda + mu read, 6 stresses written) and 12
+// floats per (cell, axis) inside that axis' CPML slab (6 psi slots read and
+// written, 3 in each half-step).
+int seis3d_step_bytes(const seis3d_ctx* ctx, uint64_t* total, uint64_t* velocity, uint64_t* stress, uint64_t* n_upd,
+                      uint64_t* n_psi) {
+    return guarded([&] {
+        if (!ctx) invalid("Seis3D: null context.");
+        seis3d_ctx c = *ctx;
+        validate(c);
+        const std::uint64_t upd = std::uint64_t(c.gx - 8) * (c.gy - 8) * (c.gz - 8);
+        std::uint64_t psi_cells = 0;
+        const int pml = active_pml(c);
+        if (pml) {
+            psi_cells += std::uint64_t(2 * pml) * (c.gy - 8) * (c.gz - 8);
+            psi_cells += std::uint64_t(2 * pml) * (c.gx - 8) * (c.gz - 8);
+            psi_cells += std::uint64_t((c.free_surface ? 1 : 2) * pml) * (c.gx - 8) * (c.gy - 8);
+        }
+        const std::uint64_t v = 4 * (13 * upd + 6 * psi_cells);
+        const std::uint64_t s = 4 * (17 * upd + 6 * psi_cells);
+        if (total) *total = v + s;
+        if (velocity) *velocity = v;
+        if (stress) *stress = s;
+        if (n_upd) *n_upd = upd;
+        if (n_psi) *n_psi = psi_cells;
+    });
+}
+
+const char* seis3d_last_error(void) { return g_error.c_str(); }
+
+const char* seis3d_version(void) { return "seiswave-mi355x 0.1.0 (gfx950, HIP)"; }
+
+}  // extern "C"

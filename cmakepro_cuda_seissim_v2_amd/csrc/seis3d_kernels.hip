@@ -1,0 +1,770 @@
+// seis3d_kernels.hip - CDNA4 (gfx950) kernels of the 3D O(8) velocity-stress
+// step: fused velocity and stress half-steps (CPML psi memories and source
+// injection fused in), free-surface ghost planes, reflecting walls,
+// receivers, materials and snapshots.
+//
+// Two kernel families compute the same arithmetic in the same order:
+//  * direct: one thread per updated cell, global loads only (the reference's
+//    decomposition of velocityKernel3D / stressKernel3D, cuda/Cuda_Seis3D.cu:222-310);
+//  * stream (2.5D): a 64 x BY workgroup owns an x-y tile and marches along z
+//    through a chunk of planes.  Each plane of every wavefield the half-step
+//    reads is staged once into LDS (x and/or y halo of 4) at the lead plane
+//    z+4; its x/y derivatives are filtered through CPML there and kept in a
+//    5-deep register queue, while the z-derivatives come from 9-deep register
+//    queues of the tile's own columns.  One pass over the wavefields per
+//    half-step; the blockIdx -> tile mapping keeps neighbouring tiles on one
+//    XCD so halos hit the same L2.
+// Built with -ffp-contract=off: every multiply and add rounds separately,
+// as in the FP32 CPU oracle.
+#include "seis3d_device.hpp"
+#include "seis3d_kernels.hpp"
+
+namespace seis3d {
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ long long cell(const DevGrid& g, int x, int y, int z) {
+    return (static_cast<long long>(z) * g.gy + y) * g.px + x;
+}
+
+template <bool plus>
+__device__ __forceinline__ float deriv_global(const float* __restrict__ f, long long i, long long s, float invh) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = f[i + (j - (plus ? 3 : 4)) * s];
+    return stencil8(v, invh);
+}
+
+// Source weight index r^2 for a cell inside the 3x3x3 box, -1 outside.
+__device__ __forceinline__ int source_r2(const DevSource& s, int x, int y, int z) {
+    const int dx = x - s.x, dy = y - s.y, dz = z - s.z;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return -1;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// ---------------------------------------------------------------------------
+// Direct family
+// ---------------------------------------------------------------------------
+
+// velocityKernel3D (cuda/Cuda_Seis3D.cu:222-245) with the force source of
+// sourceKernel3D (:326-330) fused as an epilogue.
+__global__ __launch_bounds__(256) void velocity_direct(DevStep P) {
+    const DevGrid& g = P.g;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    const int z = int(blockIdx.z) + 4;
+    if (x >= g.gx - 4 || y >= g.gy - 4 || z >= g.gz - 4) return;
+    const long long i = cell(g, x, y, z);
+    const long long st[3] = {1, g.px, g.plane};
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+    const int sf[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
+    const int r2 = P.src.active ? source_r2(P.src, x, y, z) : -1;
+    bool bad = false;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        float div = 0.0f;
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            const float* fld = P.p.f[sf[v][axis]];
+            float d = axis == v ? deriv_global<true>(fld, i, st[axis], P.invh)
+                                : deriv_global<false>(fld, i, st[axis], P.invh);
+            const int half = axis == v ? 1 : 0;
+            if (axis == 0 && qx >= 0)
+                d = cpml_update(P.p.psi[0], psi_index_x(g, v, qx, y, z), P.p.coef[0] + (half * g.gx + x) * 3, d);
+            if (axis == 1 && qy >= 0)
+                d = cpml_update(P.p.psi[1], psi_index_y(g, v, qy, x, z), P.p.coef[1] + (half * g.gy + y) * 3, d);
+            if (axis == 2 && qz >= 0)
+                d = cpml_update(P.p.psi[2], psi_index_z(g, v, qz, x, y), P.p.coef[2] + (half * g.gz + z) * 3, d);
+            div += d;
+        }
+        const float h = face_buoyancy(P.p.inv_rho[i], P.p.inv_rho[i + st[v]]);
+        float value = P.p.f[v][i] + P.dt * h * div;
+        if (r2 >= 0) value += (P.src.scale * P.src.w[r2] / P.src.norm) * P.src.dir[v] * h;
+        P.p.f[v][i] = value;
+        bad |= !isfinite(value);
+    }
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+// stressKernel3D (cuda/Cuda_Seis3D.cu:281-310) with the explosion source of
+// sourceKernel3D (:323-324) fused as an epilogue.
+__global__ __launch_bounds__(256) void stress_direct(DevStep P) {
+    const DevGrid& g = P.g;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    const int z = int(blockIdx.z) + 4;
+    if (x >= g.gx - 4 || y >= g.gy - 4 || z >= g.gz - 4) return;
+    const long long i = cell(g, x, y, z);
+    const long long st[3] = {1, g.px, g.plane};
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+    float d[3][3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float raw = a == v ? deriv_global<false>(P.p.f[v], i, st[a], P.invh)
+                               : deriv_global<true>(P.p.f[v], i, st[a], P.invh);
+            const int half = a != v ? 1 : 0;
+            if (a == 0 && qx >= 0)
+                raw = cpml_update(P.p.psi[0], psi_index_x(g, 3 + v, qx, y, z), P.p.coef[0] + (half * g.gx + x) * 3, raw);
+            if (a == 1 && qy >= 0)
+                raw = cpml_update(P.p.psi[1], psi_index_y(g, 3 + v, qy, x, z), P.p.coef[1] + (half * g.gy + y) * 3, raw);
+            if (a == 2 && qz >= 0)
+                raw = cpml_update(P.p.psi[2], psi_index_z(g, 3 + v, qz, x, y), P.p.coef[2] + (half * g.gz + z) * 3, raw);
+            d[v][a] = raw;
+        }
+    }
+    const float lam = P.p.lambda[i], mu = P.p.mu[i];
+    const bool surface = g.free_surface && z == 4;
+    if (surface) d[2][2] = -lam / (lam + 2 * mu) * (d[0][0] + d[1][1]);
+    const float trace = d[0][0] + d[1][1] + d[2][2];
+    const float* m = P.p.mu;
+    float sxx = P.p.f[3][i] + P.dt * (lam * trace + 2 * mu * d[0][0]);
+    float syy = P.p.f[4][i] + P.dt * (lam * trace + 2 * mu * d[1][1]);
+    float szz = surface ? 0.0f : P.p.f[5][i] + P.dt * (lam * trace + 2 * mu * d[2][2]);
+    const float sxy = P.p.f[6][i] + P.dt * shear_harmonic(m[i], m[i + st[0]], m[i + st[1]], m[i + st[0] + st[1]]) * (d[0][1] + d[1][0]);
+    const float syz = P.p.f[7][i] + P.dt * shear_harmonic(m[i], m[i + st[1]], m[i + st[2]], m[i + st[1] + st[2]]) * (d[1][2] + d[2][1]);
+    const float sxz = P.p.f[8][i] + P.dt * shear_harmonic(m[i], m[i + st[0]], m[i + st[2]], m[i + st[0] + st[2]]) * (d[0][2] + d[2][0]);
+    if (P.src.active) {
+        const int r2 = source_r2(P.src, x, y, z);
+        if (r2 >= 0) {
+            const float value = P.src.scale * P.src.w[r2] / P.src.norm;
+            sxx += value; syy += value; szz += value;
+        }
+    }
+    P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
+    P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
+    const bool bad = !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) || !isfinite(sxz);
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Stream (2.5D) family
+// ---------------------------------------------------------------------------
+
+constexpr int TX = 64;     // one wavefront across x
+constexpr int HX = TX + 8; // x extent with a 4-cell halo each side
+
+struct TileMap {
+    int nbx, nby, nbz;     // tiles per axis
+    int nblocks;           // nbx * nby * nbz
+    int per_xcd;           // launched blocks / 8
+    int zchunk;            // planes per tile
+};
+
+// XCD-aware tile assignment: the hardware deals workgroup b to XCD b % 8, so
+// logical tile (b % 8) * per_xcd + b / 8 gives each XCD a contiguous run of
+// x-fastest tiles (neighbours share halos in that XCD's L2).
+__device__ __forceinline__ bool tile_of(const TileMap& m, int& bx, int& by, int& bz) {
+    const int b = int(blockIdx.x);
+    const int logical = (b & 7) * m.per_xcd + (b >> 3);
+    if (logical >= m.nblocks) return false;
+    bx = logical % m.nbx;
+    const int t = logical / m.nbx;
+    by = t % m.nby;
+    bz = t / m.nby;
+    return true;
+}
+
+// Cooperative load of a ROWS x COLS window of one plane into LDS (row
+// stride COLS).  Coordinates are clamped into the padded grid: clamped
+// values only feed lanes that store nothing.
+template <int ROWS, int COLS, int NT>
+__device__ __forceinline__ void stage(float* __restrict__ lds, const float* __restrict__ f, const DevGrid& g,
+                                      int xs, int ys, int z, int tid) {
+    const long long base = static_cast<long long>(z) * g.plane;
+#pragma unroll
+    for (int k0 = 0; k0 < ROWS * COLS; k0 += NT) {
+        const int k = k0 + tid;
+        if (ROWS * COLS % NT == 0 || k < ROWS * COLS) {
+            const int r = k / COLS, c = k - r * COLS;
+            const int xx = min(xs + c, g.gx - 1), yy = min(ys + r, g.gy - 1);
+            lds[k] = f[base + static_cast<long long>(yy) * g.px + xx];
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void shift(float (&q)[N], float lead) {
+#pragma unroll
+    for (int j = 0; j < N - 1; ++j) q[j] = q[j + 1];
+    q[N - 1] = lead;
+}
+
+// Fused velocity half-step, 2.5D.  Staged at the lead plane L = z+4:
+// sxx (x halo), sxy (x+y halo), syy (y halo), sxz (x halo), syz (y halo);
+// szz needs only its own column.
+template <int BY>
+__global__ __launch_bounds__(TX * BY) void velocity_stream(DevStep P, TileMap M) {
+    constexpr int NT = TX * BY;
+    constexpr int HY = BY + 8;
+    __shared__ float s_xx[BY * HX];
+    __shared__ float s_xy[HY * HX];
+    __shared__ float s_yy[HY * TX];
+    __shared__ float s_xz[BY * HX];
+    __shared__ float s_yz[HY * TX];
+
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    const DevGrid& g = P.g;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = 4 + bz * M.zchunk;
+    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    const bool active = x < g.gx - 4 && y < g.gy - 4;
+    const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const float invh = P.invh;
+
+    float q_xz[9], q_yz[9], q_zz[9];       // columns z-4..z+4
+    float p_vx[5], p_vy[5], p_vz[5];       // x+y filtered partial sums, planes z..z+4
+
+    const float* sxx = P.p.f[3];
+    const float* syy = P.p.f[4];
+    const float* szz = P.p.f[5];
+    const float* sxy = P.p.f[6];
+    const float* syz = P.p.f[7];
+    const float* sxz = P.p.f[8];
+
+    // Stage plane L, push its centres into the queues and, for an owned
+    // plane, the CPML-filtered x/y derivative sums into the partial queues.
+    auto stage_plane = [&](int L, bool owned) {
+        __syncthreads();
+        stage<BY, HX, NT>(s_xx, sxx, g, x0 - 4, y0, L, tid);
+        stage<HY, HX, NT>(s_xy, sxy, g, x0 - 4, y0 - 4, L, tid);
+        stage<HY, TX, NT>(s_yy, syy, g, x0, y0 - 4, L, tid);
+        stage<BY, HX, NT>(s_xz, sxz, g, x0 - 4, y0, L, tid);
+        stage<HY, TX, NT>(s_yz, syz, g, x0, y0 - 4, L, tid);
+        const float lead_zz = szz[cell(g, xc, yc, L)];
+        __syncthreads();
+        shift(q_xz, s_xz[ty * HX + tx + 4]);
+        shift(q_yz, s_yz[(ty + 4) * TX + tx]);
+        shift(q_zz, lead_zz);
+        float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
+        if (owned && active) {
+            float w[8];
+            // vx: D+x sxx (half), D-y sxy.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_xx[ty * HX + tx + 1 + j];
+            float d = stencil8(w, invh);
+            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 0, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d);
+            pvx += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_xy[(ty + j) * HX + tx + 4];
+            d = stencil8(w, invh);
+            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 0, qy, x, L), P.p.coef[1] + y * 3, d);
+            pvx += d;
+            // vy: D-x sxy, D+y syy (half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_xy[(ty + 4) * HX + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 1, qx, y, L), P.p.coef[0] + x * 3, d);
+            pvy += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_yy[(ty + 1 + j) * TX + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 1, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d);
+            pvy += d;
+            // vz: D-x sxz, D-y syz.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_xz[ty * HX + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 2, qx, y, L), P.p.coef[0] + x * 3, d);
+            pvz += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_yz[(ty + j) * TX + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 2, qy, x, L), P.p.coef[1] + y * 3, d);
+            pvz += d;
+        }
+        shift(p_vx, pvx);
+        shift(p_vy, pvy);
+        shift(p_vz, pvz);
+    };
+
+#pragma unroll
+    for (int j = 0; j < 9; ++j) { q_xz[j] = 0.0f; q_yz[j] = 0.0f; q_zz[j] = 0.0f; }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { p_vx[j] = 0.0f; p_vy[j] = 0.0f; p_vz[j] = 0.0f; }
+    // Prologue: columns z0-4..z0-1 (centres only), planes z0..z0+3 staged.
+    for (int L = z0 - 4; L < z0; ++L) {
+        shift(q_xz, sxz[cell(g, xc, yc, L)]);
+        shift(q_yz, syz[cell(g, xc, yc, L)]);
+        shift(q_zz, szz[cell(g, xc, yc, L)]);
+    }
+    for (int L = z0; L < z0 + 4; ++L) stage_plane(L, L < z1);
+
+    bool bad = false;
+    for (int z = z0; z < z1; ++z) {
+        stage_plane(z + 4, z + 4 < z1);
+        // Queues now hold columns z-4..z+4 and partials of planes z..z+4.
+        if (active) {
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            const long long i = cell(g, x, y, z);
+            const float* ib = P.p.inv_rho;
+            const float a = ib[i];
+            const float hx = face_buoyancy(a, ib[i + 1]);
+            const float hy = face_buoyancy(a, ib[i + g.px]);
+            const float hz = face_buoyancy(a, ib[i + g.plane]);
+            float w[8];
+            // vx: D-z sxz (columns z-4..z+3).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_xz[j];
+            float dz = stencil8(w, invh);
+            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 0, qz, x, y), P.p.coef[2] + z * 3, dz);
+            const float div_x = p_vx[0] + dz;
+            // vy: D-z syz.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_yz[j];
+            dz = stencil8(w, invh);
+            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 1, qz, x, y), P.p.coef[2] + z * 3, dz);
+            const float div_y = p_vy[0] + dz;
+            // vz: D+z szz (columns z-3..z+4, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_zz[j + 1];
+            dz = stencil8(w, invh);
+            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 2, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, dz);
+            const float div_z = p_vz[0] + dz;
+
+            float vx = P.p.f[0][i] + P.dt * hx * div_x;
+            float vy = P.p.f[1][i] + P.dt * hy * div_y;
+            float vz = P.p.f[2][i] + P.dt * hz * div_z;
+            if (P.src.active) {
+                const int r2 = source_r2(P.src, x, y, z);
+                if (r2 >= 0) {
+                    const float value = P.src.scale * P.src.w[r2] / P.src.norm;
+                    vx += value * P.src.dir[0] * hx;
+                    vy += value * P.src.dir[1] * hy;
+                    vz += value * P.src.dir[2] * hz;
+                }
+            }
+            P.p.f[0][i] = vx; P.p.f[1][i] = vy; P.p.f[2][i] = vz;
+            bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
+        }
+    }
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+// Fused stress half-step, 2.5D.  vx, vy, vz staged at L = z+4 with x and y
+// halos; their six x/y derivatives are CPML-filtered at L and queued as
+// d00, d11, d01+d10, d20, d21.
+template <int BY>
+__global__ __launch_bounds__(TX * BY) void stress_stream(DevStep P, TileMap M) {
+    constexpr int NT = TX * BY;
+    constexpr int HY = BY + 8;
+    __shared__ float s_vx[HY * HX];
+    __shared__ float s_vy[HY * HX];
+    __shared__ float s_vz[HY * HX];
+
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    const DevGrid& g = P.g;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = 4 + bz * M.zchunk;
+    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    const bool active = x < g.gx - 4 && y < g.gy - 4;
+    const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const float invh = P.invh;
+
+    float q_vx[9], q_vy[9], q_vz[9];
+    float p00[5], p11[5], p01[5], p20[5], p21[5];
+
+    const float* fvx = P.p.f[0];
+    const float* fvy = P.p.f[1];
+    const float* fvz = P.p.f[2];
+
+    auto stage_plane = [&](int L, bool owned) {
+        __syncthreads();
+        stage<HY, HX, NT>(s_vx, fvx, g, x0 - 4, y0 - 4, L, tid);
+        stage<HY, HX, NT>(s_vy, fvy, g, x0 - 4, y0 - 4, L, tid);
+        stage<HY, HX, NT>(s_vz, fvz, g, x0 - 4, y0 - 4, L, tid);
+        __syncthreads();
+        const int c = (ty + 4) * HX + tx + 4;
+        shift(q_vx, s_vx[c]);
+        shift(q_vy, s_vy[c]);
+        shift(q_vz, s_vz[c]);
+        float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
+        if (owned && active) {
+            float w[8];
+            // vx: D-x (slot 3, integer), D+y (slot 3, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vx[(ty + 4) * HX + tx + j];
+            d00 = stencil8(w, invh);
+            if (qx >= 0) d00 = cpml_update(P.p.psi[0], psi_index_x(g, 3, qx, y, L), P.p.coef[0] + x * 3, d00);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vx[(ty + 1 + j) * HX + tx + 4];
+            float d01 = stencil8(w, invh);
+            if (qy >= 0) d01 = cpml_update(P.p.psi[1], psi_index_y(g, 3, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d01);
+            // vy: D+x (slot 4, half), D-y (slot 4, integer).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vy[(ty + 4) * HX + tx + 1 + j];
+            float d10 = stencil8(w, invh);
+            if (qx >= 0) d10 = cpml_update(P.p.psi[0], psi_index_x(g, 4, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d10);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vy[(ty + j) * HX + tx + 4];
+            d11 = stencil8(w, invh);
+            if (qy >= 0) d11 = cpml_update(P.p.psi[1], psi_index_y(g, 4, qy, x, L), P.p.coef[1] + y * 3, d11);
+            s01 = d01 + d10;
+            // vz: D+x (slot 5, half), D+y (slot 5, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vz[(ty + 4) * HX + tx + 1 + j];
+            d20 = stencil8(w, invh);
+            if (qx >= 0) d20 = cpml_update(P.p.psi[0], psi_index_x(g, 5, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d20);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s_vz[(ty + 1 + j) * HX + tx + 4];
+            d21 = stencil8(w, invh);
+            if (qy >= 0) d21 = cpml_update(P.p.psi[1], psi_index_y(g, 5, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d21);
+        }
+        shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
+    };
+
+#pragma unroll
+    for (int j = 0; j < 9; ++j) { q_vx[j] = 0.0f; q_vy[j] = 0.0f; q_vz[j] = 0.0f; }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { p00[j] = 0.0f; p11[j] = 0.0f; p01[j] = 0.0f; p20[j] = 0.0f; p21[j] = 0.0f; }
+    for (int L = z0 - 4; L < z0; ++L) {
+        shift(q_vx, fvx[cell(g, xc, yc, L)]);
+        shift(q_vy, fvy[cell(g, xc, yc, L)]);
+        shift(q_vz, fvz[cell(g, xc, yc, L)]);
+    }
+    for (int L = z0; L < z0 + 4; ++L) stage_plane(L, L < z1);
+
+    bool bad = false;
+    for (int z = z0; z < z1; ++z) {
+        stage_plane(z + 4, z + 4 < z1);
+        if (active) {
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            const long long i = cell(g, x, y, z);
+            float w[8];
+            // D+z vx, D+z vy: columns z-3..z+4 (half); D-z vz: z-4..z+3.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vx[j + 1];
+            float d02 = stencil8(w, invh);
+            if (qz >= 0) d02 = cpml_update(P.p.psi[2], psi_index_z(g, 3, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, d02);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vy[j + 1];
+            float d12 = stencil8(w, invh);
+            if (qz >= 0) d12 = cpml_update(P.p.psi[2], psi_index_z(g, 4, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, d12);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vz[j];
+            float d22 = stencil8(w, invh);
+            if (qz >= 0) d22 = cpml_update(P.p.psi[2], psi_index_z(g, 5, qz, x, y), P.p.coef[2] + z * 3, d22);
+
+            const float* m = P.p.mu;
+            const float lam = P.p.lambda[i], mu = m[i];
+            const bool surface = g.free_surface && z == 4;
+            const float d00 = p00[0], d11 = p11[0];
+            if (surface) d22 = -lam / (lam + 2 * mu) * (d00 + d11);
+            const float trace = d00 + d11 + d22;
+            const long long sx = 1, sy = g.px, sz = g.plane;
+            float sxx = P.p.f[3][i] + P.dt * (lam * trace + 2 * mu * d00);
+            float syy = P.p.f[4][i] + P.dt * (lam * trace + 2 * mu * d11);
+            float szz = surface ? 0.0f : P.p.f[5][i] + P.dt * (lam * trace + 2 * mu * d22);
+            const float mx = m[i + sx], my = m[i + sy], mz = m[i + sz];
+            const float sxy = P.p.f[6][i] + P.dt * shear_harmonic(mu, mx, my, m[i + sx + sy]) * p01[0];
+            const float syz = P.p.f[7][i] + P.dt * shear_harmonic(mu, my, mz, m[i + sy + sz]) * (d12 + p21[0]);
+            const float sxz = P.p.f[8][i] + P.dt * shear_harmonic(mu, mx, mz, m[i + sx + sz]) * (d02 + p20[0]);
+            if (P.src.active) {
+                const int r2 = source_r2(P.src, x, y, z);
+                if (r2 >= 0) {
+                    const float value = P.src.scale * P.src.w[r2] / P.src.norm;
+                    sxx += value; syy += value; szz += value;
+                }
+            }
+            P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
+            P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
+            bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) || !isfinite(sxz);
+        }
+    }
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary planes
+// ---------------------------------------------------------------------------
+
+// stressGhostKernel3D (cuda/Cuda_Seis3D.cu:203-219).
+__global__ void stress_ghost(DevStep P) {
+    const DevGrid& g = P.g;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    if (x >= g.gx - 4 || y >= g.gy - 4) return;
+    float* const* f = P.p.f;
+    f[5][cell(g, x, y, 4)] = 0.0f;
+#pragma unroll
+    for (int depth = 1; depth <= 4; ++depth) {
+        const long long ghost = cell(g, x, y, 4 - depth), mirror = cell(g, x, y, 4 + depth);
+        const long long shear_mirror = cell(g, x, y, 3 + depth);
+        f[5][ghost] = -f[5][mirror];
+        f[8][ghost] = -f[8][shear_mirror];
+        f[7][ghost] = -f[7][shear_mirror];
+        f[3][ghost] = f[3][mirror];
+        f[4][ghost] = f[4][mirror];
+        f[6][ghost] = f[6][mirror];
+    }
+}
+
+// velocityGhostKernel3D (cuda/Cuda_Seis3D.cu:249-272).
+__global__ void velocity_ghost(DevStep P, float spacing) {
+    const DevGrid& g = P.g;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    if (x >= g.gx - 4 || y >= g.gy - 4) return;
+    float* const* f = P.p.f;
+    const long long i = cell(g, x, y, 4);
+    const float invh = P.invh;
+    const float ex = deriv_global<false>(f[0], i, 1, invh);
+    const float ey = deriv_global<false>(f[1], i, g.px, invh);
+    const long long next = i + g.plane;
+    const float dz_vx = -(1.5f * deriv_global<true>(f[2], i, 1, invh) - 0.5f * deriv_global<true>(f[2], next, 1, invh));
+    const float dz_vy = -(1.5f * deriv_global<true>(f[2], i, g.px, invh) - 0.5f * deriv_global<true>(f[2], next, g.px, invh));
+    const float lam = P.p.lambda[i], mu = P.p.mu[i];
+    const float dz_vz = -lam / (lam + 2 * mu) * (ex + ey);
+#pragma unroll
+    for (int depth = 1; depth <= 4; ++depth) {
+        const long long ghost = cell(g, x, y, 4 - depth), mirror = cell(g, x, y, 4 + depth);
+        f[0][ghost] = f[0][mirror] - float(2 * depth) * spacing * dz_vx;
+        f[1][ghost] = f[1][mirror] - float(2 * depth) * spacing * dz_vy;
+        f[2][ghost] = f[2][cell(g, x, y, 3 + depth)] - float(2 * depth - 1) * spacing * dz_vz;
+    }
+}
+
+// reflectingGhostKernel3D (cuda/Cuda_Seis3D.cu:152-175), one axis per launch.
+__global__ void reflecting_walls(DevStep P, int axis, int velocities) {
+    const DevGrid& g = P.g;
+    const int lengths[3] = {g.gx, g.gy, g.gz};
+    const int u_axis = (axis + 1) % 3, v_axis = (axis + 2) % 3;
+    const int u = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int v = int(blockIdx.y * blockDim.y + threadIdx.y);
+    if (u >= lengths[u_axis] || v >= lengths[v_axis]) return;
+    const int high = lengths[axis] - 4 - 1;
+    int p[3] = {0, 0, 0};
+    p[u_axis] = u; p[v_axis] = v;
+    for (int field = velocities ? 0 : 3; field < (velocities ? 3 : 9); ++field) {
+        const bool half = field < 3 ? field == axis : field == 6 ? axis != 2 : field == 7 ? axis != 0
+                        : field == 8 ? axis != 1 : false;
+        float* f = P.p.f[field];
+        for (int side = 0; side < 2; ++side) for (int depth = 1; depth <= 4; ++depth) {
+            p[axis] = side ? high + depth : 4 - depth;
+            const long long destination = cell(g, p[0], p[1], p[2]);
+            if (half && !side && depth == 1) { f[destination] = 0.0f; continue; }
+            p[axis] = side ? high + 1 - depth - int(half) : 4 + depth - 1 - int(half);
+            const float value = f[cell(g, p[0], p[1], p[2])];
+            f[destination] = half ? -value : value;
+        }
+        if (half) { p[axis] = high; f[cell(g, p[0], p[1], p[2])] = 0.0f; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Receivers, materials, snapshots
+// ---------------------------------------------------------------------------
+
+// sampleScalar3D (cuda/Cuda_Seis3D.cu:335-352) at a padded index.
+__device__ float sample_scalar(const DevStep& P, long long i, int component) {
+    const DevGrid& g = P.g;
+    float* const* f = P.p.f;
+    if (component == 5) return sqrtf((P.p.lambda[i] + 2 * P.p.mu[i]) * P.p.inv_rho[i]);
+    if (component == 6) return sqrtf(P.p.mu[i] * P.p.inv_rho[i]);
+    if (component == 7) return 1 / P.p.inv_rho[i];
+    const float vx = 0.5f * (f[0][i] + f[0][i - 1]);
+    const float vy = 0.5f * (f[1][i] + f[1][i - g.px]);
+    const float vz = 0.5f * (f[2][i] + f[2][i - g.plane]);
+    if (component == 0) return vz;
+    if (component == 1) return vx;
+    if (component == 2) return vy;
+    if (component == 3) return sqrtf(vx * vx + vy * vy + vz * vz);
+    if (component == 4) return -(f[3][i] + f[4][i] + f[5][i]) / 3;
+    return 0.0f;
+}
+
+// receiverKernel3D (cuda/Cuda_Seis3D.cu:421-428): out[3r..3r+2] = vx, vy, vz.
+// Receivers arrive as padded coordinates.
+__global__ void receivers(DevStep P, const int* __restrict__ rec, int count, float* __restrict__ out) {
+    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= count) return;
+    const long long i = cell(P.g, rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+    out[3 * r] = sample_scalar(P, i, 1);
+    out[3 * r + 1] = sample_scalar(P, i, 2);
+    out[3 * r + 2] = sample_scalar(P, i, 0);
+}
+
+// generateMaterialsKernel3D (cuda/Cuda_Seis3D.cu:114-147).
+__global__ void generate_materials(DevStep P, MaterialParams mp) {
+    const DevGrid& g = P.g;
+    const long long cells = static_cast<long long>(g.gx) * g.gy * g.gz;
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= cells) return;
+    const int gxv = int(k % g.gx), gyv = int((k / g.gx) % g.gy), gzv = int(k / (static_cast<long long>(g.gx) * g.gy));
+    const int x = max(0, min(mp.nx - 1, gxv - mp.ox));
+    const int y = max(0, min(mp.ny - 1, gyv - mp.oy));
+    const int z = max(0, min(mp.nz - 1, gzv - mp.oz));
+    const float dx = (x - mp.nx * 0.5f) / mp.nx;
+    const float dy = (y - mp.ny * 0.5f) / mp.ny;
+    const float dz = (z - mp.nz * 0.55f) / mp.nz;
+    const float depth = float(z) / float(mp.nz - 1), horizontal = float(x) / float(mp.nx - 1) - 0.5f;
+    float blend = 0.0f;
+    switch (mp.kind) {
+    case 1: blend = depth >= mp.interface_depth; break;
+    case 2: blend = dx * dx + dy * dy + dz * dz < mp.anomaly_radius * mp.anomaly_radius; break;
+    case 4: blend = depth < 1.0f / 3 ? 0 : depth < 2.0f / 3 ? 0.5f : 1; break;
+    case 5: blend = depth >= mp.interface_depth + mp.layer_dip * horizontal; break;
+    case 6: blend = fabsf(depth - mp.interface_depth) <= 0.5f * mp.layer_thickness; break;
+    case 7: blend = depth >= mp.interface_depth + mp.layer_dip * horizontal + (horizontal > 0 ? mp.fault_throw : 0); break;
+    case 8: blend = depth; break;
+    case 9: {
+        const float vertical = (depth - 0.78f) / 0.52f;
+        blend = (dx * dx + dy * dy) / (mp.anomaly_radius * mp.anomaly_radius) + vertical * vertical < 1;
+        break;
+    }
+    default: break;
+    }
+    const float vp = mp.vp + blend * (mp.layer_vp - mp.vp), vs = mp.vs + blend * (mp.layer_vs - mp.vs);
+    const float rho = mp.rho + blend * (mp.layer_rho - mp.rho);
+    const long long i = cell(g, gxv, gyv, gzv);
+    P.p.inv_rho[i] = 1.0f / rho;
+    P.p.mu[i] = rho * vs * vs;
+    P.p.lambda[i] = rho * (vp * vp - 2.0f * vs * vs);
+}
+
+// extendMaterialsKernel3D (cuda/Cuda_Seis3D.cu:188-200).
+__global__ void extend_materials(DevStep P, MaterialParams mp) {
+    const DevGrid& g = P.g;
+    const long long cells = static_cast<long long>(g.gx) * g.gy * g.gz;
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= cells) return;
+    const int x = int(k % g.gx), y = int((k / g.gx) % g.gy), z = int(k / (static_cast<long long>(g.gx) * g.gy));
+    const int px = max(mp.ox, min(x, mp.ox + mp.nx - 1));
+    const int py = max(mp.oy, min(y, mp.oy + mp.ny - 1));
+    const int pz = max(mp.oz, min(z, mp.oz + mp.nz - 1));
+    if (x == px && y == py && z == pz) return;
+    const long long i = cell(g, x, y, z), j = cell(g, px, py, pz);
+    P.p.inv_rho[i] = P.p.inv_rho[j];
+    P.p.mu[i] = P.p.mu[j];
+    P.p.lambda[i] = P.p.lambda[j];
+}
+
+// Packed slice of one collocated component over the display domain
+// (extractKernel3D slice part, cuda/Cuda_Seis3D.cu:370-394, without GL).
+__global__ void extract_slice(DevStep P, SliceParams sp, float* __restrict__ out) {
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<long long>(sp.width) * sp.height) return;
+    const int u = int(k % sp.width), v = int(k / sp.width);
+    int p[3];
+    p[sp.fixed_axis] = sp.fixed;
+    p[sp.u_axis] = u + sp.u_origin;
+    p[sp.v_axis] = v + sp.v_origin;
+    const long long i = cell(P.g, p[0] + sp.ox, p[1] + sp.oy, p[2] + sp.oz);
+    const float value = sample_scalar(P, i, sp.component);
+    out[k] = isfinite(value) ? value : 0.0f;
+    if (!isfinite(value)) atomicOr(P.p.flags, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static hipError_t stream_launch(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {
+    TileMap m;
+    m.nbx = (P.g.gx - 8 + TX - 1) / TX;
+    m.nby = (P.g.gy - 8 + tile_y - 1) / tile_y;
+    m.zchunk = zchunk;
+    m.nbz = (P.g.gz - 8 + zchunk - 1) / zchunk;
+    m.nblocks = m.nbx * m.nby * m.nbz;
+    const int launched = (m.nblocks + 7) / 8 * 8;
+    m.per_xcd = launched / 8;
+    const dim3 grid(launched);
+    switch (tile_y) {
+    case 4:
+        if (velocity) hipLaunchKernelGGL(velocity_stream<4>, grid, dim3(TX, 4), 0, s, P, m);
+        else hipLaunchKernelGGL(stress_stream<4>, grid, dim3(TX, 4), 0, s, P, m);
+        break;
+    case 8:
+        if (velocity) hipLaunchKernelGGL(velocity_stream<8>, grid, dim3(TX, 8), 0, s, P, m);
+        else hipLaunchKernelGGL(stress_stream<8>, grid, dim3(TX, 8), 0, s, P, m);
+        break;
+    case 16:
+        if (velocity) hipLaunchKernelGGL(velocity_stream<16>, grid, dim3(TX, 16), 0, s, P, m);
+        else hipLaunchKernelGGL(stress_stream<16>, grid, dim3(TX, 16), 0, s, P, m);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s) {
+    if (k.family == 1) return stream_launch(true, P, k.tile_y, k.z_chunk, s);
+    const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
+    hipLaunchKernelGGL(velocity_direct, grid, block, 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s) {
+    if (k.family == 1) return stream_launch(false, P, k.tile_y, k.z_chunk, s);
+    const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
+    hipLaunchKernelGGL(stress_direct, grid, block, 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s) {
+    const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4);
+    hipLaunchKernelGGL(stress_ghost, grid, block, 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s) {
+    const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4);
+    hipLaunchKernelGGL(velocity_ghost, grid, block, 0, s, P, spacing);
+    return hipGetLastError();
+}
+
+hipError_t launch_reflecting_walls(const DevStep& P, bool velocities, hipStream_t s) {
+    const int lengths[3] = {P.g.gx, P.g.gy, P.g.gz};
+    const dim3 block(16, 16);
+    for (int axis = 0; axis < 3; ++axis) {
+        const dim3 grid((lengths[(axis + 1) % 3] + 15) / 16, (lengths[(axis + 2) % 3] + 15) / 16);
+        hipLaunchKernelGGL(reflecting_walls, grid, block, 0, s, P, axis, velocities ? 1 : 0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_receivers(const DevStep& P, const int* rec, int count, float* out, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(receivers, dim3((count + 63) / 64), dim3(64), 0, s, P, rec, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s) {
+    const long long cells = static_cast<long long>(P.g.gx) * P.g.gy * P.g.gz;
+    hipLaunchKernelGGL(generate_materials, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, s, P, mp);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s) {
+    const long long cells = static_cast<long long>(P.g.gx) * P.g.gy * P.g.gz;
+    hipLaunchKernelGGL(extend_materials, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, s, P, mp);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_slice(const DevStep& P, const SliceParams& sp, float* out, hipStream_t s) {
+    const long long n = static_cast<long long>(sp.width) * sp.height;
+    hipLaunchKernelGGL(extract_slice, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, P, sp, out);
+    return hipGetLastError();
+}
+
+}  // namespace seis3d

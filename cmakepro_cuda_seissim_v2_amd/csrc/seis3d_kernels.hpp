@@ -1,0 +1,44 @@
+// seis3d_kernels.hpp - host-visible launchers of the 3D kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "seis3d_device.hpp"
+
+namespace seis3d {
+
+// Kernel family and tile shape of the two fused half-steps.
+struct KernelConfig {
+    int family = 1;    // 0 direct, 1 stream (2.5D)
+    int tile_y = 8;    // rows per workgroup (stream): 4, 8 or 16
+    int z_chunk = 32;  // planes per workgroup (stream)
+};
+
+// Procedural model parameters of generateMaterialsKernel3D.
+struct MaterialParams {
+    int nx, ny, nz, ox, oy, oz;
+    int kind;
+    float vp, vs, rho, layer_vp, layer_vs, layer_rho;
+    float interface_depth, layer_dip, layer_thickness, fault_throw, anomaly_radius;
+};
+
+// One packed slice: p[fixed_axis] = fixed, p[u_axis] = u + u_origin,
+// p[v_axis] = v + v_origin in physical coordinates, then + (ox, oy, oz).
+struct SliceParams {
+    int fixed_axis, fixed, u_axis, v_axis;
+    int u_origin, v_origin, width, height;
+    int ox, oy, oz;
+    int component;
+};
+
+hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s);
+hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s);
+hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s);
+hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s);
+hipError_t launch_reflecting_walls(const DevStep& P, bool velocities, hipStream_t s);
+hipError_t launch_receivers(const DevStep& P, const int* rec, int count, float* out, hipStream_t s);
+hipError_t launch_generate_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);
+hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);
+hipError_t launch_extract_slice(const DevStep& P, const SliceParams& sp, float* out, hipStream_t s);
+
+}  // namespace seis3d

@@ -1,0 +1,506 @@
+"""Python host mirror of the Seis3D solver-session API over the C ABI (include/seis3d.h).
+
+Names follow the reference's operator interface (cuda/Cuda_Seis3D.cuh:6-42):
+``validateContext3D``, ``estimateMemory3D``, ``initializeGpu3D``, ``releaseGpu3D``,
+``resetGpu3D``, ``refreshCpmlCoefficients3D``, ``runStep3D``, ``setReceivers3D``,
+``sampleReceivers3D``, ``probeMaterial3D``, ``uploadModelPlane3D``,
+``finishModelUpload3D``, ``downloadModelPlane3D``; error behaviour maps the C codes
+back to the reference's exception classes (invalid_argument -> ``InvalidArgument``
+(a ValueError), runtime_error -> ``SolverRuntimeError``, logic_error -> ``LogicError``).
+
+All compute runs in the HIP library ``lib/libseis3d.so``; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libseis3d.so"
+
+VX, VY, VZ, SXX, SYY, SZZ, SXY, SYZ, SXZ = range(9)
+INV_RHO, LAMBDA, MU = 9, 10, 11
+EXPLOSION, FORCE = 0, 1
+(UNIFORM, TWO_LAYER, SPHERE, IMPORTED, THREE_LAYER, DIPPING_LAYER, LOW_VELOCITY_LAYER, FAULT,
+ GRADIENT, SALT_DOME) = range(10)
+MAX_RECEIVERS = 300
+
+
+class InvalidArgument(ValueError):
+    """std::invalid_argument of the reference (error code 1)."""
+
+
+class SolverRuntimeError(RuntimeError):
+    """std::runtime_error of the reference (error codes 2 and 4)."""
+
+
+class LogicError(RuntimeError):
+    """std::logic_error of the reference (error code 3)."""
+
+
+class NonFiniteError(SolverRuntimeError):
+    """Non-finite wavefield detected (error code 4)."""
+
+
+class Ctx(C.Structure):
+    """seis3d_ctx = SimulationContext3D (include/Common3D.h:24-42)."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pml", C.c_int32),
+        ("free_surface", C.c_int32), ("reflecting", C.c_int32),
+        ("spacing", C.c_float), ("cfl", C.c_float), ("dt", C.c_float),
+        ("manual_dt", C.c_int32), ("requested_dt_ms", C.c_float), ("safe_dt", C.c_float),
+        ("vp", C.c_float), ("vs", C.c_float), ("density", C.c_float),
+        ("layer_vp", C.c_float), ("layer_vs", C.c_float), ("layer_density", C.c_float),
+        ("max_vp", C.c_float), ("frequency", C.c_float),
+        ("model_kind", C.c_int32),
+        ("interface_depth", C.c_float), ("layer_dip", C.c_float), ("layer_thickness", C.c_float),
+        ("fault_throw", C.c_float), ("anomaly_radius", C.c_float),
+        ("gx", C.c_int32), ("gy", C.c_int32), ("gz", C.c_int32),
+        ("ox", C.c_int32), ("oy", C.c_int32), ("oz", C.c_int32),
+        ("cells", C.c_uint64),
+    ]
+
+    def copy(self) -> "Ctx":
+        out = Ctx()
+        C.pointer(out)[0] = self
+        return out
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Source(C.Structure):
+    """seis3d_source = Source3D (include/Common3D.h:44-49)."""
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("kind", C.c_int32),
+                ("amplitude", C.c_float), ("direction", C.c_float * 3)]
+
+
+class Receiver(C.Structure):
+    """seis3d_receiver = Receiver3D (include/Common3D.h:51)."""
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32)]
+
+
+class Budget(C.Structure):
+    """seis3d_budget = MemoryBudget3D (include/Common3D.h:72-74)."""
+    _fields_ = [("field_bytes", C.c_uint64), ("cpml_bytes", C.c_uint64), ("view_bytes", C.c_uint64),
+                ("total_bytes", C.c_uint64)]
+
+
+class Survey(C.Structure):
+    """seis3d_survey = the spread part of Survey3D (include/Seis3D_Sim.h:36-45)."""
+    _fields_ = [("receiver_count_x", C.c_int32), ("receiver_count_y", C.c_int32),
+                ("receiver_origin", C.c_int32 * 3), ("receiver_spacing", C.c_int32 * 2),
+                ("shot_count_x", C.c_int32), ("shot_count_y", C.c_int32),
+                ("shot_origin", C.c_int32 * 3), ("shot_spacing", C.c_int32 * 2)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("velocity_ms", C.c_double), ("stress_ms", C.c_double), ("total_ms", C.c_double),
+                ("launches", C.c_int64)]
+
+
+_LIB = None
+
+
+def library() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"HIP solver library missing: {LIB_PATH} (run __graft_entry__.build())")
+        lib = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        vp = C.c_void_p
+        sig = {
+            "seis3d_default_ctx": (None, [P(Ctx)]),
+            "seis3d_default_source": (None, [P(Source)]),
+            "seis3d_default_survey": (None, [P(Survey)]),
+            "seis3d_validate_ctx": (C.c_int, [P(Ctx)]),
+            "seis3d_estimate_memory": (C.c_int, [P(Ctx), P(Budget)]),
+            "seis3d_device_bytes": (C.c_int, [P(Ctx), P(C.c_uint64)]),
+            "seis3d_cpml_coefficients": (C.c_int, [P(Ctx), C.c_int, P(C.c_float), C.c_size_t]),
+            "seis3d_fit_survey": (C.c_int, [P(Ctx), P(Survey)]),
+            "seis3d_survey_receivers": (C.c_int, [P(Ctx), P(Survey), P(Receiver), P(C.c_int)]),
+            "seis3d_sequence_shot": (C.c_int, [P(Ctx), P(Survey), C.c_int, P(Source)]),
+            "seis3d_gather_timing": (C.c_int, [P(Ctx), C.c_int, P(C.c_int), P(C.c_double)]),
+            "seis3d_ricker": (C.c_float, [C.c_float, C.c_double]),
+            "seis3d_create": (C.c_int, [P(Ctx), P(vp)]),
+            "seis3d_destroy": (None, [vp]),
+            "seis3d_get_ctx": (C.c_int, [vp, P(Ctx)]),
+            "seis3d_reset": (C.c_int, [vp]),
+            "seis3d_refresh_cpml": (C.c_int, [vp, P(Ctx)]),
+            "seis3d_read_cpml": (C.c_int, [vp, C.c_int, P(C.c_float), C.c_size_t]),
+            "seis3d_upload_plane": (C.c_int, [vp, C.c_int, P(C.c_float)]),
+            "seis3d_finish_upload": (C.c_int, [vp]),
+            "seis3d_download_plane": (C.c_int, [vp, C.c_int, P(C.c_float)]),
+            "seis3d_probe": (C.c_int, [vp, P(Receiver), P(C.c_float)]),
+            "seis3d_step": (C.c_int, [vp, P(Source), C.c_uint64]),
+            "seis3d_run": (C.c_int, [vp, P(Source), C.c_uint64, C.c_int64]),
+            "seis3d_synchronize": (C.c_int, [vp]),
+            "seis3d_set_receivers": (C.c_int, [vp, P(Receiver), C.c_int]),
+            "seis3d_receiver_count": (C.c_int, [vp]),
+            "seis3d_sample_receivers": (C.c_int, [vp, P(P(C.c_float))]),
+            "seis3d_gather_begin": (C.c_int, [vp, C.c_int, C.c_int]),
+            "seis3d_gather_count": (C.c_int, [vp]),
+            "seis3d_gather_read": (C.c_int, [vp, P(C.c_float), C.c_size_t, P(C.c_int)]),
+            "seis3d_gather_end": (C.c_int, [vp]),
+            "seis3d_download_field": (C.c_int, [vp, C.c_int, P(C.c_float)]),
+            "seis3d_upload_field": (C.c_int, [vp, C.c_int, P(C.c_float)]),
+            "seis3d_extract_slice": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), C.c_size_t]),
+            "seis3d_set_tiles": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+            "seis3d_get_tiles": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+            "seis3d_run_timed": (C.c_int, [vp, P(Source), C.c_uint64, C.c_int64, P(Timing)]),
+            "seis3d_step_bytes": (C.c_int, [P(Ctx), P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                                            P(C.c_uint64), P(C.c_uint64)]),
+            "seis3d_last_error": (C.c_char_p, []),
+            "seis3d_version": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    """C symbols this wrapper binds (must match include/seis3d.h)."""
+    library()
+    return sorted(n for n in dir(_LIB) if n.startswith("seis3d_"))
+
+
+def _check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = library().seis3d_last_error().decode(errors="replace")
+    if rc == 1:
+        raise InvalidArgument(msg)
+    if rc == 3:
+        raise LogicError(msg)
+    if rc == 4:
+        raise NonFiniteError(msg)
+    raise SolverRuntimeError(msg)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ---- context helpers -------------------------------------------------------
+
+def default_context(**overrides) -> Ctx:
+    """SimulationContext3D{} defaults (include/Common3D.h:24-42) with overrides."""
+    c = Ctx()
+    library().seis3d_default_ctx(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise AttributeError(k)
+        setattr(c, k, v)
+    return c
+
+
+def default_source(**overrides) -> Source:
+    s = Source()
+    library().seis3d_default_source(C.byref(s))
+    for k, v in overrides.items():
+        if k == "direction":
+            for i in range(3):
+                s.direction[i] = float(v[i])
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def default_survey(**overrides) -> Survey:
+    q = Survey()
+    library().seis3d_default_survey(C.byref(q))
+    for k, v in overrides.items():
+        f = getattr(q, k)
+        if isinstance(f, C.Array):
+            for i, x in enumerate(v):
+                f[i] = x
+        else:
+            setattr(q, k, v)
+    return q
+
+
+def validateContext3D(ctx: Ctx) -> Ctx:
+    """validateContext3D (cuda/Cuda_Seis3D.cu:433-483): derives dims and dt in place."""
+    _check(library().seis3d_validate_ctx(C.byref(ctx)))
+    return ctx
+
+
+def estimateMemory3D(ctx: Ctx) -> Budget:
+    b = Budget()
+    _check(library().seis3d_estimate_memory(C.byref(ctx), C.byref(b)))
+    return b
+
+
+def device_bytes(ctx: Ctx) -> int:
+    v = C.c_uint64()
+    _check(library().seis3d_device_bytes(C.byref(ctx), C.byref(v)))
+    return int(v.value)
+
+
+def makeCoefficients3D(ctx: Ctx, axis: int) -> np.ndarray:
+    n = (ctx.gx, ctx.gy, ctx.gz)[axis]
+    out = np.zeros(n * 6, np.float32)
+    _check(library().seis3d_cpml_coefficients(C.byref(ctx), axis, _fptr(out), out.size))
+    return out
+
+
+def fitSurvey3D(ctx: Ctx, survey: Survey | None = None) -> Survey:
+    q = survey if survey is not None else default_survey()
+    _check(library().seis3d_fit_survey(C.byref(ctx), C.byref(q)))
+    return q
+
+
+def configureReceivers3D(ctx: Ctx, survey: Survey) -> list[tuple[int, int, int]]:
+    buf = (Receiver * MAX_RECEIVERS)()
+    n = C.c_int(MAX_RECEIVERS)
+    _check(library().seis3d_survey_receivers(C.byref(ctx), C.byref(survey), buf, C.byref(n)))
+    return [(buf[i].x, buf[i].y, buf[i].z) for i in range(n.value)]
+
+
+def positionSequenceShot3D(ctx: Ctx, survey: Survey, index: int, src: Source) -> Source:
+    _check(library().seis3d_sequence_shot(C.byref(ctx), C.byref(survey), index, C.byref(src)))
+    return src
+
+
+def gather_timing(ctx: Ctx, stride: int) -> tuple[int, float]:
+    iv = C.c_int()
+    t0 = C.c_double()
+    _check(library().seis3d_gather_timing(C.byref(ctx), stride, C.byref(iv), C.byref(t0)))
+    return iv.value, t0.value
+
+
+def step_bytes(ctx: Ctx) -> dict:
+    vals = [C.c_uint64() for _ in range(5)]
+    _check(library().seis3d_step_bytes(C.byref(ctx), *[C.byref(v) for v in vals]))
+    keys = ("total", "velocity", "stress", "n_upd", "n_psi")
+    return {k: int(v.value) for k, v in zip(keys, vals)}
+
+
+def ricker(frequency: float, time: float) -> float:
+    return float(library().seis3d_ricker(frequency, time))
+
+
+def version() -> str:
+    return library().seis3d_version().decode()
+
+
+# ---- session ---------------------------------------------------------------
+
+class GPUSimData3D:
+    """One solver session (GPUSimData3D of include/Common3D.h:76-93) on the current HIP device."""
+
+    def __init__(self, ctx: Ctx | None = None):
+        self._h = C.c_void_p()
+        self.ctx: Ctx | None = None
+        if ctx is not None:
+            self.initialize(ctx)
+
+    # initializeGpu3D (cuda/Cuda_Seis3D.cu:519-555)
+    def initialize(self, ctx: Ctx) -> "GPUSimData3D":
+        if self._h.value:
+            raise LogicError("Seis3D: release the previous session before initializing.")
+        h = C.c_void_p()
+        _check(library().seis3d_create(C.byref(ctx), C.byref(h)))
+        self._h = h
+        self.ctx = ctx.copy()
+        return self
+
+    # releaseGpu3D (cuda/Cuda_Seis3D.cu:558-569): idempotent
+    def release(self) -> None:
+        if self._h.value:
+            library().seis3d_destroy(self._h)
+        self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def _ctx_now(self) -> Ctx:
+        c = Ctx()
+        _check(library().seis3d_get_ctx(self._h, C.byref(c)))
+        return c
+
+    def reset(self) -> None:
+        _check(library().seis3d_reset(self._h))
+
+    def refresh_cpml(self, ctx: Ctx) -> None:
+        _check(library().seis3d_refresh_cpml(self._h, C.byref(ctx)))
+        self.ctx = self._ctx_now()
+
+    def read_cpml(self, axis: int) -> np.ndarray:
+        n = (self.ctx.gx, self.ctx.gy, self.ctx.gz)[axis]
+        out = np.zeros(n * 6, np.float32)
+        _check(library().seis3d_read_cpml(self._h, axis, _fptr(out), out.size))
+        return out
+
+    def upload_plane(self, z: int, vp_vs_rho: np.ndarray) -> None:
+        a = np.ascontiguousarray(vp_vs_rho, dtype=np.float32)
+        if a.size != self.ctx.nx * self.ctx.ny * 3:
+            raise InvalidArgument("Seis3D: invalid model plane.")
+        _check(library().seis3d_upload_plane(self._h, int(z), _fptr(a)))
+
+    def finish_upload(self) -> None:
+        _check(library().seis3d_finish_upload(self._h))
+
+    def upload_model(self, vp: np.ndarray, vs: np.ndarray, rho: np.ndarray) -> None:
+        """Upload a whole physical model [nz, ny, nx] plane by plane, then extend."""
+        for z in range(self.ctx.nz):
+            plane = np.stack([vp[z], vs[z], rho[z]], axis=-1).astype(np.float32)
+            self.upload_plane(z, plane)
+        self.finish_upload()
+
+    def download_plane(self, z: int) -> np.ndarray:
+        out = np.zeros(self.ctx.nx * self.ctx.ny * 3, np.float32)
+        _check(library().seis3d_download_plane(self._h, int(z), _fptr(out)))
+        return out.reshape(self.ctx.ny, self.ctx.nx, 3)
+
+    def probe(self, x: int, y: int, z: int) -> np.ndarray:
+        vals = np.zeros(3, np.float32)
+        p = Receiver(x, y, z)
+        _check(library().seis3d_probe(self._h, C.byref(p), _fptr(vals)))
+        return vals
+
+    def step(self, src: Source, step: int) -> None:
+        _check(library().seis3d_step(self._h, C.byref(src), int(step)))
+
+    def run(self, src: Source, first_step: int, n_steps: int) -> None:
+        _check(library().seis3d_run(self._h, C.byref(src), int(first_step), int(n_steps)))
+
+    def run_timed(self, src: Source, first_step: int, n_steps: int) -> Timing:
+        t = Timing()
+        _check(library().seis3d_run_timed(self._h, C.byref(src), int(first_step), int(n_steps), C.byref(t)))
+        return t
+
+    def synchronize(self) -> None:
+        _check(library().seis3d_synchronize(self._h))
+
+    def set_receivers(self, receivers) -> None:
+        rec = list(receivers)
+        arr = (Receiver * max(1, len(rec)))()
+        for i, r in enumerate(rec):
+            arr[i] = Receiver(int(r[0]), int(r[1]), int(r[2]))
+        _check(library().seis3d_set_receivers(self._h, arr if rec else None, len(rec)))
+
+    @property
+    def receiver_count(self) -> int:
+        return int(library().seis3d_receiver_count(self._h))
+
+    def sample_receivers(self) -> np.ndarray | None:
+        ptr = C.POINTER(C.c_float)()
+        _check(library().seis3d_sample_receivers(self._h, C.byref(ptr)))
+        if not ptr:
+            return None
+        n = self.receiver_count
+        return np.ctypeslib.as_array(ptr, shape=(n * 3,)).copy().reshape(n, 3)
+
+    def gather_begin(self, target_samples: int, stride: int) -> None:
+        _check(library().seis3d_gather_begin(self._h, int(target_samples), int(stride)))
+
+    @property
+    def gather_count(self) -> int:
+        return int(library().seis3d_gather_count(self._h))
+
+    def gather_read(self) -> np.ndarray:
+        n = self.gather_count
+        r = self.receiver_count
+        out = np.zeros(max(1, n * r * 3), np.float32)
+        got = C.c_int()
+        _check(library().seis3d_gather_read(self._h, _fptr(out), out.size, C.byref(got)))
+        return out[: got.value * r * 3].reshape(got.value, r, 3)
+
+    def gather_end(self) -> None:
+        _check(library().seis3d_gather_end(self._h))
+
+    def download_field(self, field: int) -> np.ndarray:
+        c = self.ctx
+        out = np.zeros(c.gz * c.gy * c.gx, np.float32)
+        _check(library().seis3d_download_field(self._h, int(field), _fptr(out)))
+        return out.reshape(c.gz, c.gy, c.gx)
+
+    def upload_field(self, field: int, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        c = self.ctx
+        if a.size != c.gx * c.gy * c.gz:
+            raise InvalidArgument("Seis3D: invalid field size.")
+        _check(library().seis3d_upload_field(self._h, int(field), _fptr(a)))
+
+    def extract_slice(self, axis: int, at: int, component: int) -> np.ndarray:
+        c = self.ctx
+        pml = 0 if c.reflecting else c.pml
+        dom = (c.nx + 2 * pml, c.ny + 2 * pml, c.nz + (1 if c.free_surface else 2) * pml)
+        u = 1 if axis == 0 else 0
+        v = 1 if axis == 2 else 2
+        out = np.zeros(dom[u] * dom[v], np.float32)
+        _check(library().seis3d_extract_slice(self._h, axis, at, component, _fptr(out), out.size))
+        return out.reshape(dom[v], dom[u])
+
+    def set_tiles(self, family: int, tile_y: int = 8, z_chunk: int = 32) -> None:
+        _check(library().seis3d_set_tiles(self._h, family, tile_y, z_chunk))
+
+    def get_tiles(self) -> tuple[int, int, int]:
+        f, y, z = C.c_int(), C.c_int(), C.c_int()
+        _check(library().seis3d_get_tiles(self._h, C.byref(f), C.byref(y), C.byref(z)))
+        return f.value, y.value, z.value
+
+
+# Reference-named free functions (cuda/Cuda_Seis3D.cuh:6-42).
+def initializeGpu3D(ctx: Ctx) -> GPUSimData3D:
+    return GPUSimData3D(ctx)
+
+
+def releaseGpu3D(g: GPUSimData3D) -> None:
+    g.release()
+
+
+def resetGpu3D(g: GPUSimData3D) -> None:
+    g.reset()
+
+
+def refreshCpmlCoefficients3D(g: GPUSimData3D, ctx: Ctx) -> None:
+    g.refresh_cpml(ctx)
+
+
+def runStep3D(g: GPUSimData3D, src: Source, step: int) -> None:
+    g.step(src, step)
+
+
+def setReceivers3D(g: GPUSimData3D, receivers) -> None:
+    g.set_receivers(receivers)
+
+
+def sampleReceivers3D(g: GPUSimData3D):
+    return g.sample_receivers()
+
+
+def probeMaterial3D(g: GPUSimData3D, x: int, y: int, z: int) -> np.ndarray:
+    return g.probe(x, y, z)
+
+
+def uploadModelPlane3D(g: GPUSimData3D, z: int, values: np.ndarray) -> None:
+    g.upload_plane(z, values)
+
+
+def finishModelUpload3D(g: GPUSimData3D) -> None:
+    g.finish_upload()
+
+
+def downloadModelPlane3D(g: GPUSimData3D, z: int) -> np.ndarray:
+    return g.download_plane(z)

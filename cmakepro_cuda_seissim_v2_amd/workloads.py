@@ -1,0 +1,80 @@
+"""Synthetic workloads of the benchmark configurations (seeded, no external data).
+
+* ``layered_model``: 5 horizontal layers with seeded random interface depths,
+  vp 1800..4500 m/s, vs = vp/1.73, Gardner density 310 * vp**0.25.
+* ``gradient_random_model``: smooth vp 2000 -> 4000 m/s depth gradient with a seeded Gaussian
+  random field (correlation length 20 cells, +-10 %).
+* ``config2_context`` / ``config2_setup``: the 300 x 50 x 300 free-surface + 20-cell CPML run
+  with a 15 Hz vertical force at (150, 25, 75) and the 150-receiver default spread.
+
+Config 2 asks for dt = 1 ms at dx = 10 m with vp_max = 4500 m/s: Courant 0.45 exceeds the
+O(8) 3D limit 0.44884 and validateContext3D (cuda/Cuda_Seis3D.cu:479-480) rejects it, so the
+runs use the reference's automatic dt (847 us at CFL 0.85).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import seis3d as S
+
+BASE_SEED = 20261203
+LAYER_VP = (1800.0, 2475.0, 3150.0, 3825.0, 4500.0)
+
+
+def gardner_density(vp: np.ndarray) -> np.ndarray:
+    return (310.0 * np.power(vp, 0.25)).astype(np.float32)
+
+
+def layered_model(nx: int, ny: int, nz: int, seed: int = BASE_SEED):
+    """Five flat layers; interface depths drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.choice(np.arange(2, nz - 2), size=len(LAYER_VP) - 1, replace=False))
+    layer = np.searchsorted(cuts, np.arange(nz), side="right")
+    vp_z = np.asarray(LAYER_VP, np.float32)[layer]
+    vp = np.broadcast_to(vp_z[:, None, None], (nz, ny, nx)).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    return vp, vs, gardner_density(vp)
+
+
+def gaussian_random_field(shape, corr: float, seed: int) -> np.ndarray:
+    """FFT-filtered white noise with a Gaussian kernel of length `corr` cells, scaled to |g| <= 1."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) for n in shape]
+    kk = sum(np.meshgrid(*[ki ** 2 for ki in k], indexing="ij"))
+    filt = np.exp(-0.5 * (2 * np.pi) ** 2 * kk * corr ** 2)
+    g = np.real(np.fft.ifftn(np.fft.fftn(noise) * filt))
+    return (g / np.max(np.abs(g))).astype(np.float32)
+
+
+def gradient_random_model(nx: int, ny: int, nz: int, seed: int = BASE_SEED, corr: float = 20.0,
+                          amplitude: float = 0.1):
+    z = np.arange(nz, dtype=np.float32) / max(1, nz - 1)
+    bg = (2000.0 + 2000.0 * z)[:, None, None]
+    g = gaussian_random_field((nz, ny, nx), corr, seed)
+    vp = (bg * (1.0 + amplitude * g)).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    return vp, vs, gardner_density(vp)
+
+
+def imported_context(nx: int, ny: int, nz: int, max_vp: float, pml: int = 20, free_surface: bool = True,
+                     spacing: float = 10.0, frequency: float = 15.0, **kw) -> S.Ctx:
+    c = S.default_context(nx=nx, ny=ny, nz=nz, pml=pml, free_surface=int(free_surface), spacing=spacing,
+                          frequency=frequency, model_kind=S.IMPORTED, max_vp=float(max_vp), **kw)
+    return S.validateContext3D(c)
+
+
+def config2_context(nx: int = 300, ny: int = 50, nz: int = 300) -> S.Ctx:
+    return imported_context(nx, ny, nz, max_vp=max(LAYER_VP))
+
+
+def config2_source(ctx: S.Ctx) -> S.Source:
+    """15 Hz vertical force at the reference's default source position (nx/2, ny/2, nz/4)."""
+    return S.default_source(x=ctx.nx // 2, y=ctx.ny // 2, z=max(2, ctx.nz // 4), kind=S.FORCE,
+                            amplitude=1.0e8, direction=(0.0, 0.0, 1.0))
+
+
+def config2_receivers(ctx: S.Ctx):
+    """The default 50 x 3 surface spread of fitSurvey3D/configureReceivers3D."""
+    q = S.fitSurvey3D(ctx, S.default_survey())
+    return S.configureReceivers3D(ctx, q)

@@ -1,0 +1,241 @@
+/*
+ * seis3d.h - C ABI of the MI355X-native 3D elastic velocity-stress solver.
+ *
+ * Every entry point replaces one function of the reference's Seis3D solver
+ * API (cuda/Cuda_Seis3D.cuh:6-42 on the POD types of include/Common3D.h:24-98)
+ * or one step of its session driver (src/Seis3D_Sim.cpp,
+ * src/Seis3D/Acquisition3D.cpp).  The reference API is a set of C++ free
+ * functions that throw; here every function returns an error code and keeps a
+ * thread-local message (seis3d_last_error).  include/Seis3D.hpp maps the codes
+ * back to the reference's exception types and message prefixes.
+ *
+ * Error codes: 0 ok, 1 invalid argument (std::invalid_argument), 2 HIP/runtime
+ * failure (std::runtime_error), 3 lifecycle misuse (std::logic_error),
+ * 4 non-finite wavefield detected (std::runtime_error).
+ *
+ * Layouts at the boundary are the reference's: padded grid gx*gy*gz with x
+ * fastest (include/Common3D.h:96-98); material planes are interleaved SI
+ * triples [Vp,Vs,rho] with x fastest (cuda/Cuda_Seis3D.cu:696-711); receiver
+ * samples are [receiver][vx,vy,vz] (cuda/Cuda_Seis3D.cu:421-428) and gathers
+ * [sample][receiver][vx,vy,vz] (include/Seis3D_Sim.h:52).
+ */
+#ifndef SEIS3D_H
+#define SEIS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum seis3d_status {
+    SEIS3D_OK = 0,
+    SEIS3D_EINVAL = 1,
+    SEIS3D_EHIP = 2,
+    SEIS3D_ELIFECYCLE = 3,
+    SEIS3D_ENONFINITE = 4
+};
+
+/* Field3D (include/Common3D.h:18). */
+enum seis3d_field {
+    SEIS3D_VX = 0, SEIS3D_VY, SEIS3D_VZ,
+    SEIS3D_SXX, SEIS3D_SYY, SEIS3D_SZZ, SEIS3D_SXY, SEIS3D_SYZ, SEIS3D_SXZ,
+    /* Material arrays, readable through seis3d_download_field. */
+    SEIS3D_INV_RHO = 9, SEIS3D_LAMBDA = 10, SEIS3D_MU = 11
+};
+
+/* ModelKind3D (include/Common3D.h:20-21), same numeric IDs. */
+enum seis3d_model_kind {
+    SEIS3D_MODEL_UNIFORM = 0, SEIS3D_MODEL_TWO_LAYER, SEIS3D_MODEL_SPHERE, SEIS3D_MODEL_IMPORTED,
+    SEIS3D_MODEL_THREE_LAYER, SEIS3D_MODEL_DIPPING_LAYER, SEIS3D_MODEL_LOW_VELOCITY_LAYER,
+    SEIS3D_MODEL_FAULT, SEIS3D_MODEL_GRADIENT, SEIS3D_MODEL_SALT_DOME
+};
+
+/* SourceKind3D (include/Common3D.h:22). */
+enum seis3d_source_kind { SEIS3D_EXPLOSION = 0, SEIS3D_FORCE = 1 };
+
+/* SimulationContext3D (include/Common3D.h:24-42); bools are int32 0/1. */
+typedef struct seis3d_ctx {
+    int32_t nx, ny, nz;
+    int32_t pml;
+    int32_t free_surface;
+    int32_t reflecting;
+    float spacing, cfl, dt;
+    int32_t manual_dt;
+    float requested_dt_ms;
+    float safe_dt;
+    float vp, vs, density;
+    float layer_vp, layer_vs, layer_density;
+    float max_vp, frequency;
+    int32_t model_kind;
+    float interface_depth, layer_dip, layer_thickness;
+    float fault_throw, anomaly_radius;
+    /* Derived by seis3d_validate_ctx only. */
+    int32_t gx, gy, gz, ox, oy, oz;
+    uint64_t cells;
+} seis3d_ctx;
+
+/* Source3D (include/Common3D.h:44-49). */
+typedef struct seis3d_source {
+    int32_t x, y, z;
+    int32_t kind;
+    float amplitude;
+    float direction[3];
+} seis3d_source;
+
+/* Receiver3D (include/Common3D.h:51). */
+typedef struct seis3d_receiver { int32_t x, y, z; } seis3d_receiver;
+
+/* MemoryBudget3D (include/Common3D.h:72-74). */
+typedef struct seis3d_budget {
+    uint64_t field_bytes, cpml_bytes, view_bytes, total_bytes;
+} seis3d_budget;
+
+/* Survey3D receiver/shot spread (include/Seis3D_Sim.h:36-45). */
+typedef struct seis3d_survey {
+    int32_t receiver_count_x, receiver_count_y;
+    int32_t receiver_origin[3];
+    int32_t receiver_spacing[2];
+    int32_t shot_count_x, shot_count_y;
+    int32_t shot_origin[3];
+    int32_t shot_spacing[2];
+} seis3d_survey;
+
+/* Per-kernel timing of a run (HIP events on the session stream). */
+typedef struct seis3d_timing {
+    double velocity_ms;  /* summed duration of the fused velocity half-step kernel */
+    double stress_ms;    /* summed duration of the fused stress half-step kernel */
+    double total_ms;     /* whole run, first launch to last completion */
+    int64_t launches;    /* launches of each of the two kernels */
+} seis3d_timing;
+
+typedef struct seis3d_session seis3d_t;
+
+/* ---- Context, validation, budgets (host only, no GPU needed) ---------- */
+
+/* Defaults of SimulationContext3D{} (include/Common3D.h:24-42). */
+void seis3d_default_ctx(seis3d_ctx* ctx);
+/* Defaults of Source3D{} (include/Common3D.h:44-49). */
+void seis3d_default_source(seis3d_source* src);
+/* Defaults of Survey3D{} (include/Seis3D_Sim.h:36-45). */
+void seis3d_default_survey(seis3d_survey* survey);
+/* validateContext3D (cuda/Cuda_Seis3D.cu:433-483). */
+int seis3d_validate_ctx(seis3d_ctx* ctx);
+/* estimateMemory3D (cuda/Cuda_Seis3D.cu:504-516): the reference's formula. */
+int seis3d_estimate_memory(const seis3d_ctx* ctx, seis3d_budget* out);
+/* Device bytes this library allocates for ctx (fields, materials, CPML). */
+int seis3d_device_bytes(const seis3d_ctx* ctx, uint64_t* out);
+/* makeCoefficients3D (cuda/Cuda_Seis3D.cu:45-64): [half][n][a,b,1/kappa]. */
+int seis3d_cpml_coefficients(const seis3d_ctx* ctx, int axis, float* out, size_t capacity);
+/* fitSurvey3D (src/Seis3D_Sim.cpp:42-60). */
+int seis3d_fit_survey(const seis3d_ctx* ctx, seis3d_survey* survey);
+/* configureReceivers3D layout (src/Seis3D/Acquisition3D.cpp:21-37); *count in/out. */
+int seis3d_survey_receivers(const seis3d_ctx* ctx, const seis3d_survey* survey,
+                            seis3d_receiver* out, int* count);
+/* positionSequenceShot3D (src/Seis3D/Acquisition3D.cpp:233-242). */
+int seis3d_sequence_shot(const seis3d_ctx* ctx, const seis3d_survey* survey, int index,
+                         seis3d_source* src);
+/* Gather timing of startGather3D (src/Seis3D/Acquisition3D.cpp:201-209). */
+int seis3d_gather_timing(const seis3d_ctx* ctx, int stride, int* interval_us,
+                         double* first_sample_time);
+/* Ricker sample of sourceKernel3D (cuda/Cuda_Seis3D.cu:319-320) at time t. */
+float seis3d_ricker(float frequency, double time);
+
+/* ---- Session lifetime (initializeGpu3D / releaseGpu3D) ------------------ */
+
+/* initializeGpu3D (cuda/Cuda_Seis3D.cu:519-555) on the current HIP device.
+ * ctx must have passed seis3d_validate_ctx. */
+int seis3d_create(const seis3d_ctx* ctx, seis3d_t** out);
+/* releaseGpu3D (cuda/Cuda_Seis3D.cu:558-569); NULL is accepted. */
+void seis3d_destroy(seis3d_t* t);
+/* The context the session was created with (after any refresh). */
+int seis3d_get_ctx(const seis3d_t* t, seis3d_ctx* out);
+/* resetGpu3D (cuda/Cuda_Seis3D.cu:572-577). */
+int seis3d_reset(seis3d_t* t);
+/* refreshCpmlCoefficients3D (cuda/Cuda_Seis3D.cu:580-599). */
+int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx);
+/* Read back the device CPML table of one axis ([half][n][a,b,1/kappa]). */
+int seis3d_read_cpml(seis3d_t* t, int axis, float* out, size_t capacity);
+
+/* ---- Model ---------------------------------------------------------------- */
+
+/* uploadModelPlane3D (cuda/Cuda_Seis3D.cu:696-711). */
+int seis3d_upload_plane(seis3d_t* t, int z, const float* vp_vs_rho);
+/* finishModelUpload3D (cuda/Cuda_Seis3D.cu:714-717). */
+int seis3d_finish_upload(seis3d_t* t);
+/* downloadModelPlane3D (cuda/Cuda_Seis3D.cu:720-733). */
+int seis3d_download_plane(seis3d_t* t, int z, float* vp_vs_rho);
+/* probeMaterial3D (cuda/Cuda_Seis3D.cu:684-693); out of range leaves values. */
+int seis3d_probe(seis3d_t* t, const seis3d_receiver* p, float values[3]);
+
+/* ---- Stepping ------------------------------------------------------------- */
+
+/* runStep3D (cuda/Cuda_Seis3D.cu:602-624): one leapfrog step, enqueue only. */
+int seis3d_step(seis3d_t* t, const seis3d_source* src, uint64_t step);
+/* n_steps runStep3D calls from first_step, enqueue only.  When a gather is
+ * recording, samples are taken after every step s with (s+1) % stride == 0
+ * into the device gather, with no host synchronisation
+ * (advanceSimulation3D, src/Seis3D_Sim.cpp:120-153). */
+int seis3d_run(seis3d_t* t, const seis3d_source* src, uint64_t first_step, int64_t n_steps);
+/* Block until the session stream is idle; reports the non-finite flag. */
+int seis3d_synchronize(seis3d_t* t);
+
+/* ---- Receivers and gathers ------------------------------------------------ */
+
+/* setReceivers3D (cuda/Cuda_Seis3D.cu:649-668): validated atomic replace. */
+int seis3d_set_receivers(seis3d_t* t, const seis3d_receiver* receivers, int count);
+int seis3d_receiver_count(const seis3d_t* t);
+/* sampleReceivers3D (cuda/Cuda_Seis3D.cu:671-681): [r][vx,vy,vz] in owned
+ * pinned memory, valid until the next sample, set_receivers or destroy;
+ * *out is NULL when no receivers are set. */
+int seis3d_sample_receivers(seis3d_t* t, const float** out);
+/* startGather3D recording state (src/Seis3D/Acquisition3D.cpp:188-230):
+ * allocate a device gather of target_samples x receivers x 3 and sample
+ * every stride steps during seis3d_run. */
+int seis3d_gather_begin(seis3d_t* t, int target_samples, int stride);
+/* Samples recorded so far (host counter, no synchronisation). */
+int seis3d_gather_count(const seis3d_t* t);
+/* Copy the recorded [sample][receiver][3] gather to host; checks the
+ * non-finite flag first (error 4, as sampleReceivers3D). */
+int seis3d_gather_read(seis3d_t* t, float* out, size_t capacity_floats, int* samples);
+/* Stop recording and release the device gather. */
+int seis3d_gather_end(seis3d_t* t);
+
+/* ---- Wavefield access (snapshots; replaces the GL interop views) --------- */
+
+/* Copy one padded field (seis3d_field) to host in the reference layout
+ * gx*gy*gz, x fastest.  Synchronises the session stream. */
+int seis3d_download_field(seis3d_t* t, int field, float* out);
+/* Overwrite one padded wavefield (0..8) from a host array in that layout. */
+int seis3d_upload_field(seis3d_t* t, int field, const float* in);
+/* Physical-domain XZ (axis 1), YZ (axis 0) or XY (axis 2) slice at physical
+ * index `at`, collocated component as sampleScalar3D (cuda/Cuda_Seis3D.cu:335-352):
+ * 0 vz, 1 vx, 2 vy, 3 |v|, 4 pressure, 5 vp, 6 vs, 7 rho.  The slice covers
+ * the display domain of makeViewLayout3D (CPML included, ghosts excluded). */
+int seis3d_extract_slice(seis3d_t* t, int axis, int at, int component, float* out, size_t capacity);
+
+/* ---- Tuning and measurement ---------------------------------------------- */
+
+/* Kernel family: 0 = direct (one thread per cell), 1 = 2.5D LDS/register
+ * streaming.  tile_y rows per block, z_chunk planes per block (family 1). */
+int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk);
+int seis3d_get_tiles(const seis3d_t* t, int* family, int* tile_y, int* z_chunk);
+/* seis3d_run with HIP events around every fused-kernel launch. */
+int seis3d_run_timed(seis3d_t* t, const seis3d_source* src, uint64_t first_step,
+                     int64_t n_steps, seis3d_timing* timing);
+/* Algorithmic HBM bytes of one step: 4*(30*N_upd + 12*N_psi), with the
+ * velocity and stress kernel shares. */
+int seis3d_step_bytes(const seis3d_ctx* ctx, uint64_t* total, uint64_t* velocity,
+                      uint64_t* stress, uint64_t* n_upd, uint64_t* n_psi);
+
+/* ---- Errors and build info ------------------------------------------------ */
+
+const char* seis3d_last_error(void);
+const char* seis3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEIS3D_H */

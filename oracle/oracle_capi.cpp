@@ -1,0 +1,182 @@
+// oracle_capi.cpp - extern "C" entry points of the CPU oracle for the Python
+// tests and the cpu_baseline leg of bench.py (loaded with ctypes).
+//
+// TEST INFRASTRUCTURE ONLY: the checker, never the product path.
+#include <chrono>
+#include <cstdio>
+#include <exception>
+#include <memory>
+#include <string>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "seis3d_oracle.hpp"
+
+namespace {
+thread_local std::string g_error;
+
+struct Handle {
+    int precision = 32;
+    std::unique_ptr<seis3d_oracle::Solver<float>> f32;
+    std::unique_ptr<seis3d_oracle::Solver<double>> f64;
+};
+
+template <class F>
+int guard(F&& fn) {
+    try { fn(); return 0; }
+    catch (const std::invalid_argument& e) { g_error = e.what(); return 1; }
+    catch (const std::exception& e) { g_error = e.what(); return 2; }
+}
+
+template <class Fn>
+auto with_solver(Handle* h, Fn&& fn) {
+    if (h->precision == 32) return fn(*h->f32);
+    return fn(*h->f64);
+}
+}  // namespace
+
+extern "C" {
+
+const char* oracle3d_last_error() { return g_error.c_str(); }
+
+int oracle3d_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int oracle3d_validate(seis3d_ctx* ctx) { return guard([&] { seis3d_oracle::validate(*ctx); }); }
+
+int oracle3d_coefficients(const seis3d_ctx* ctx, int axis, float* out) {
+    return guard([&] {
+        const auto v = seis3d_oracle::make_coefficients<float>(*ctx, axis);
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+uint64_t oracle3d_slab_count(const seis3d_ctx* ctx, int axis) { return seis3d_oracle::slab_count(*ctx, axis); }
+
+double oracle3d_ricker(float frequency, double time) { return seis3d_oracle::ricker_double(frequency, time); }
+
+int oracle3d_fit_survey(const seis3d_ctx* ctx, seis3d_survey* s) {
+    return guard([&] { seis3d_oracle::fit_survey(*ctx, *s); });
+}
+
+int oracle3d_survey_receivers(const seis3d_ctx* ctx, const seis3d_survey* s, seis3d_receiver* out, int* count) {
+    return guard([&] {
+        const auto r = seis3d_oracle::survey_receivers(*ctx, *s);
+        if (int(r.size()) > *count) throw std::invalid_argument("receiver buffer too small");
+        std::copy(r.begin(), r.end(), out);
+        *count = int(r.size());
+    });
+}
+
+int oracle3d_create(const seis3d_ctx* ctx, int precision, Handle** out) {
+    return guard([&] {
+        auto h = std::make_unique<Handle>();
+        h->precision = precision == 64 ? 64 : 32;
+        if (h->precision == 32) h->f32 = std::make_unique<seis3d_oracle::Solver<float>>(*ctx);
+        else h->f64 = std::make_unique<seis3d_oracle::Solver<double>>(*ctx);
+        *out = h.release();
+    });
+}
+
+void oracle3d_destroy(Handle* h) { delete h; }
+
+int oracle3d_reset(Handle* h) { return guard([&] { with_solver(h, [](auto& s) { s.reset(); }); }); }
+
+int oracle3d_refresh_cpml(Handle* h, const seis3d_ctx* ctx) {
+    return guard([&] { with_solver(h, [&](auto& s) { s.refresh_cpml(*ctx); }); });
+}
+
+int oracle3d_upload_plane(Handle* h, int z, const float* values) {
+    return guard([&] { with_solver(h, [&](auto& s) { s.upload_plane(z, values); }); });
+}
+
+int oracle3d_finish_upload(Handle* h) { return guard([&] { with_solver(h, [](auto& s) { s.finish_upload(); }); }); }
+
+int oracle3d_set_receivers(Handle* h, const seis3d_receiver* r, int count) {
+    return guard([&] { with_solver(h, [&](auto& s) { s.set_receivers(r, count); }); });
+}
+
+int oracle3d_step(Handle* h, const seis3d_source* src, uint64_t step) {
+    return guard([&] { with_solver(h, [&](auto& s) { s.step(*src, step); }); });
+}
+
+// n_steps steps from first_step; after each step s with (s+1) % stride == 0
+// the receivers are sampled into gather[sample][r][3] (up to max_samples).
+int oracle3d_run(Handle* h, const seis3d_source* src, uint64_t first_step, int64_t n_steps, int stride,
+                 double* gather, int max_samples, int* samples) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            using Real = std::decay_t<decltype(s.f[0][0])>;
+            std::vector<Real> buf(s.receivers.size() * 3);
+            int count = 0;
+            for (int64_t k = 0; k < n_steps; ++k) {
+                const uint64_t step = first_step + uint64_t(k);
+                s.step(*src, step);
+                if (stride > 0 && gather && (step + 1) % uint64_t(stride) == 0 && count < max_samples) {
+                    s.sample(buf.data());
+                    for (std::size_t j = 0; j < buf.size(); ++j) gather[std::size_t(count) * buf.size() + j] = double(buf[j]);
+                    ++count;
+                }
+            }
+            if (samples) *samples = count;
+        });
+    });
+}
+
+// Wall-clock seconds of n_steps steps (the CPU baseline timing).
+double oracle3d_time_steps(Handle* h, const seis3d_source* src, uint64_t first_step, int64_t n_steps) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = oracle3d_run(h, src, first_step, n_steps, 0, nullptr, 0, nullptr);
+    const auto t1 = std::chrono::steady_clock::now();
+    return rc ? -1.0 : std::chrono::duration<double>(t1 - t0).count();
+}
+
+int oracle3d_sample(Handle* h, double* out) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            using Real = std::decay_t<decltype(s.f[0][0])>;
+            std::vector<Real> buf(s.receivers.size() * 3);
+            s.sample(buf.data());
+            for (std::size_t j = 0; j < buf.size(); ++j) out[j] = double(buf[j]);
+        });
+    });
+}
+
+// field 0..8 wavefields, 9 1/rho, 10 lambda, 11 mu; padded reference layout.
+int oracle3d_get_field(Handle* h, int field, double* out) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            const auto& v = field < 9 ? s.f[field] : field == 9 ? s.inv_rho : field == 10 ? s.lambda : s.mu;
+            for (std::size_t i = 0; i < v.size(); ++i) out[i] = double(v[i]);
+        });
+    });
+}
+
+int oracle3d_set_field(Handle* h, int field, const double* in) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            using Real = std::decay_t<decltype(s.f[0][0])>;
+            auto& v = field < 9 ? s.f[field] : field == 9 ? s.inv_rho : field == 10 ? s.lambda : s.mu;
+            for (std::size_t i = 0; i < v.size(); ++i) v[i] = Real(in[i]);
+        });
+    });
+}
+
+int oracle3d_sample_scalar(Handle* h, int x, int y, int z, int component, double* out) {
+    return guard([&] { with_solver(h, [&](auto& s) { *out = double(s.sample_scalar(x, y, z, component)); }); });
+}
+
+int oracle3d_nonfinite(Handle* h) {
+    return with_solver(h, [](auto& s) { return s.nonfinite ? 1 : 0; });
+}
+
+}  // extern "C"
