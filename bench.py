@@ -26,6 +26,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s 3D O(8) VS-FD + %HBM roofline; batched FWI-grad shots/s vs plain"
+FAMILY_NAMES = {0: "direct", 1: "stream-2.5D", 2: "pipelined-2.5D", 3: "column", 4: "lds-dma-2.5D"}
+FAMILY_KERNEL = {0: "direct", 1: "stream", 2: "pipe", 3: "column", 4: "dma"}
 
 
 def parse():
@@ -33,14 +35,29 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=50)
-    p.add_argument("--family", type=int, default=1)
+    p.add_argument("--family", type=int, default=-1, help="kernel family (-1: library default)")
     p.add_argument("--tile-y", type=int, default=0)
     p.add_argument("--z-chunk", type=int, default=0)
     p.add_argument("--kernel-steps", type=int, default=200, help="steps timed per kernel with HIP events")
     p.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", default="300x50x300")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="instead of the bench, run 5 plain device copies of MIB MiB (counter calibration)")
     return p.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """Plain device-to-device copies of a known size, used to calibrate rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters against a kernel whose HBM traffic is known."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
 
 
 def cpu_info():
@@ -87,6 +104,9 @@ def cpu_baseline(ctx, model, src, rec, steps_hint):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
@@ -103,9 +123,9 @@ def main():
     sb = S.step_bytes(ctx)
 
     g = S.GPUSimData3D(ctx)
-    if args.tile_y or args.z_chunk or args.family != 1:
+    if args.tile_y or args.z_chunk or args.family >= 0:
         f0, ty0, zc0 = g.get_tiles()
-        g.set_tiles(args.family, args.tile_y or ty0, args.z_chunk or zc0)
+        g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk or zc0)
     family, tile_y, z_chunk = g.get_tiles()
     g.upload_model(*model)
     g.set_receivers(rec)
@@ -156,7 +176,7 @@ def main():
                         f"15 Hz force, 150 receivers stride 2, dt=847 us (1 ms violates CFL)",
             "grid": [nx, ny, nz], "padded": [ctx.gx, ctx.gy, ctx.gz],
             "n_upd": n_upd, "n_psi": sb["n_psi"], "dt_us": round(ctx.dt * 1e6),
-            "kernel_family": "stream-2.5D" if family == 1 else "direct",
+            "kernel_family": FAMILY_NAMES.get(family, str(family)),
             "tile_y": tile_y, "z_chunk": z_chunk,
             "step_hbm_gbs_algorithmic": round(step_gbs, 1),
             "step_frac_of_peak": round(step_gbs / HBM_PEAK_GBS, 4),
@@ -165,7 +185,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{dominant[0]}_stream" if family == 1 else f"{dominant[0]}_direct",
+            "kernel": f"{dominant[0]}_{FAMILY_KERNEL.get(family, 'kernel')}",
             "achieved": round(dominant[3], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

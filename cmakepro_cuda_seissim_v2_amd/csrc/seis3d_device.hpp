@@ -36,6 +36,7 @@ struct DevFields {
     float* inv_rho;
     float* lambda;
     float* mu;
+    float* inv_mu;        // 1/mu per cell, rebuilt after every material change
     float* psi[3];
     const float* coef[3];
     unsigned int* flags;  // [0] non-finite flag
@@ -101,6 +102,14 @@ __device__ __forceinline__ float shear_harmonic(float m0, float m1, float m2, fl
     return (m0 > 0 && m1 > 0 && m2 > 0 && m3 > 0) ? 4 / (1 / m0 + 1 / m1 + 1 / m2 + 1 / m3) : 0.0f;
 }
 
+// shearMu3D from the reciprocals r_k = 1/mu_k, each the same IEEE division
+// the reference performs, so the result is bitwise identical: r > 0 holds for
+// every mu > 0 (r = +inf for mu = +0 or a tiny denormal, where the reference
+// also returns 4/inf = 0 or fails its test and returns 0).
+__device__ __forceinline__ float shear_from_inverse(float r0, float r1, float r2, float r3) {
+    return (r0 > 0 && r1 > 0 && r2 > 0 && r3 > 0) ? 4 / (r0 + r1 + r2 + r3) : 0.0f;
+}
+
 // derivative3D (cuda/Cuda_Seis3D.cu:78-89) on eight samples s[0..7] ordered
 // from the most negative to the most positive tap: D+ at i+1/2 uses
 // f[i-3..i+4], D- at i uses f[i-4..i+3]; either way s[4+r] - s[3-r].
@@ -111,6 +120,40 @@ __device__ __forceinline__ float stencil8(const float s[8], float invh) {
     r += kW2 * (s[6] - s[1]);
     r += kW3 * (s[7] - s[0]);
     return r * invh;
+}
+
+// ---- 2.5D streaming tiles -------------------------------------------------
+
+constexpr int TX = 64;     // one wavefront across x
+constexpr int HX = TX + 8; // x extent with a 4-cell halo each side
+
+struct TileMap {
+    int nbx, nby, nbz;     // tiles per axis
+    int nblocks;           // nbx * nby * nbz
+    int per_xcd;           // launched blocks / 8
+    int zchunk;            // planes per tile
+};
+
+// XCD-aware tile assignment: the hardware deals workgroup b to XCD b % 8, so
+// logical tile (b % 8) * per_xcd + b / 8 gives each XCD a contiguous run of
+// x-fastest tiles (neighbours share halos in that XCD's L2).
+__device__ __forceinline__ bool tile_of(const TileMap& m, int& bx, int& by, int& bz) {
+    const int b = int(blockIdx.x);
+    const int logical = (b & 7) * m.per_xcd + (b >> 3);
+    if (logical >= m.nblocks) return false;
+    bx = logical % m.nbx;
+    const int t = logical / m.nbx;
+    by = t % m.nby;
+    bz = t / m.nby;
+    return true;
+}
+
+
+template <int N>
+__device__ __forceinline__ void shift(float (&q)[N], float lead) {
+#pragma unroll
+    for (int j = 0; j < N - 1; ++j) q[j] = q[j + 1];
+    q[N - 1] = lead;
 }
 
 }  // namespace seis3d

@@ -1,0 +1,619 @@
+// seis3d_dma.hip - 2.5D velocity and stress half-steps staged by direct
+// global-to-LDS loads (gfx950 global_load_lds_dword).
+//
+// Same arithmetic, operation for operation, as velocityKernel3D /
+// stressKernel3D (cuda/Cuda_Seis3D.cu:222-310).  A 64 x BY workgroup owns an
+// x-y tile and marches along z through a chunk of planes:
+//  * every wavefield plane that needs an x or y derivative is copied from HBM
+//    straight into LDS (one 64-lane row per instruction plus an 8-lane tail
+//    for the x halo), with no VGPRs holding the data in flight;
+//  * the LDS ring has three plane buffers; the copy of plane k+2 is the last
+//    vector-memory instruction of iteration k, so at the top of iteration k+1
+//    "at most N_DMA outstanding" (N_DMA = copies per plane per wave) proves
+//    that plane k+1 has landed for this wave, and the workgroup barrier that
+//    follows proves it for every wave and frees the oldest buffer;
+//  * the per-cell operands (the fields being updated, materials, CPML
+//    memories) are loaded into registers one plane ahead;
+//  * x/y derivatives are CPML-filtered at the lead plane and queued (5 deep),
+//    z derivatives come from 9-deep register queues of the tile's columns.
+// The loads are explicit waits rather than compiler-tracked, so every wait
+// is conservative: a wave may wait for more than it needs, never less.
+#include "seis3d_device.hpp"
+#include "seis3d_kernels.hpp"
+
+namespace seis3d {
+
+namespace {
+
+__device__ __forceinline__ long long dcell(const DevGrid& g, int x, int y, int z) {
+    return (static_cast<long long>(z) * g.gy + y) * g.px + x;
+}
+
+__device__ __forceinline__ float source_weight(const DevSource& s, int r2) {
+    return r2 == 0 ? s.w[0] : r2 == 1 ? s.w[1] : r2 == 2 ? s.w[2] : s.w[3];
+}
+
+__device__ __forceinline__ int source_r2d(const DevSource& s, int x, int y, int z) {
+    const int dx = x - s.x, dy = y - s.y, dz = z - s.z;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return -1;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Wait until at most N vector-memory operations of this wave are outstanding
+// (gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Lane l of the wave copies src (its own address) to lds_row[l].
+__device__ __forceinline__ void copy_row(const float* src, float* lds_row) {
+    __builtin_amdgcn_global_load_lds(src, lds_row, 4, 0, 0);
+}
+
+// Copy ROWS rows of one plane into an LDS tile of width W (72: x halo,
+// starting at column x0 - 4; 64: starting at x0).  Wave ty copies rows
+// ty, ty + BY, ...  A 72-wide row takes two full-wave copies, columns 0..63
+// and 8..71: the overlap is written twice with identical values, and no copy
+// depends on the exec mask, so every wave issues exactly COPIES instructions.
+template <int BY, int ROWS, int W>
+struct TileCopy {
+    static constexpr int PER = (ROWS + BY - 1) / BY;
+    static constexpr int COPIES = PER * (W == 72 ? 2 : 1);
+    static_assert(ROWS % BY == 0, "rows per wave must be exact");
+
+    __device__ __forceinline__ static void issue(float* lds, const float* f, const DevGrid& g, int xs, int ys, int L,
+                                                 int tx, int ty) {
+        const long long base = static_cast<long long>(L) * g.plane;
+        const int xm = min(xs + tx, g.gx - 1);
+        const int xt = min(xs + 8 + tx, g.gx - 1);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int r = ty + k * BY;
+            const long long row = base + static_cast<long long>(min(ys + r, g.gy - 1)) * g.px;
+            copy_row(f + row + xm, lds + r * W);
+            if (W == 72) copy_row(f + row + xt, lds + r * W + 8);
+        }
+    }
+};
+
+template <int BY>
+struct VelTiles {
+    float xx[BY * 72];
+    float xy[(BY + 8) * 72];
+    float yy[(BY + 8) * 64];
+    float xz[BY * 72];
+    float yz[(BY + 8) * 64];
+};
+
+template <int BY>
+constexpr int vel_copies() {
+    return TileCopy<BY, BY, 72>::COPIES + TileCopy<BY, BY + 8, 72>::COPIES + TileCopy<BY, BY + 8, 64>::COPIES +
+           TileCopy<BY, BY, 72>::COPIES + TileCopy<BY, BY + 8, 64>::COPIES;
+}
+
+// Per-cell operands of one velocity iteration (lead plane k, update plane k-4).
+struct VelOperands {
+    float zz;          // szz at the lead plane, own column
+    float px[3], py[3];  // x- and y-slab CPML memories of the lead plane
+    float v[3];        // velocities at the update plane
+    float b[4];        // 1/rho at i, i+x, i+y, i+z
+    float pz[3];       // z-slab CPML memories at the update plane
+};
+
+constexpr int kMaxChunk = 128;   // planes per tile (z table capacity)
+
+// Copy the tile's slices of the three CPML tables ([half][n][a,b,1/kappa])
+// into LDS as [half*3 + k][index]; entries outside a slab are never read.
+__device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, int z0, int z1, int by_rows,
+                                            float* cx, float* cyt, float* czt) {
+    const DevGrid& g = P.g;
+    if (g.pml == 0) return;
+    const int tid = threadIdx.y * 64 + threadIdx.x, nt = 64 * by_rows;
+    for (int k = tid; k < 6 * 64; k += nt) {
+        const int j = k / 64, c = min(x0 + k % 64, g.gx - 1);
+        cx[k] = P.p.coef[0][((j / 3) * g.gx + c) * 3 + j % 3];
+    }
+    for (int k = tid; k < 6 * by_rows; k += nt) {
+        const int j = k / by_rows, c = min(y0 + k % by_rows, g.gy - 1);
+        cyt[k] = P.p.coef[1][((j / 3) * g.gy + c) * 3 + j % 3];
+    }
+    const int nz = z1 - z0;
+    for (int k = tid; k < 6 * nz; k += nt) {
+        const int j = k / nz, c = z0 + k % nz;
+        czt[j * kMaxChunk + k % nz] = P.p.coef[2][((j / 3) * g.gz + c) * 3 + j % 3];
+    }
+}
+
+}  // namespace
+
+// Fused velocity half-step (velocityKernel3D + force source epilogue).
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void velocity_dma(DevStep P, TileMap M) {
+    __shared__ VelTiles<BY> ring[3];
+    __shared__ float cx[6 * 64];          // x CPML table: [half*3 + k][lane]
+    __shared__ float cyt[6 * BY];         // y CPML table: [half*3 + k][row]
+    __shared__ float czt[6 * kMaxChunk];  // z CPML table: [half*3 + k][plane - z0]
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    const DevGrid& g = P.g;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = 4 + bz * M.zchunk;
+    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    const int k0 = z0 - 4, last = z1 + 3;
+    const bool active = x < g.gx - 4 && y < g.gy - 4;
+    const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const float invh = P.invh;
+    const float* sxx = P.p.f[3];
+    const float* syy = P.p.f[4];
+    const float* szz = P.p.f[5];
+    const float* sxy = P.p.f[6];
+    const float* syz = P.p.f[7];
+    const float* sxz = P.p.f[8];
+    const float* ib = P.p.inv_rho;
+    constexpr int NDMA = vel_copies<BY>();
+
+    // CPML coefficient tables of this tile (read from LDS: no vector-memory
+    // waits inside the plane loop).
+    fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
+    __syncthreads();
+
+    auto copy_plane = [&](int slot, int L) {
+        VelTiles<BY>& s = ring[(slot - k0) % 3];
+        TileCopy<BY, BY, 72>::issue(s.xx, sxx, g, x0 - 4, y0, L, tx, ty);
+        TileCopy<BY, BY + 8, 72>::issue(s.xy, sxy, g, x0 - 4, y0 - 4, L, tx, ty);
+        TileCopy<BY, BY + 8, 64>::issue(s.yy, syy, g, x0, y0 - 4, L, tx, ty);
+        TileCopy<BY, BY, 72>::issue(s.xz, sxz, g, x0 - 4, y0, L, tx, ty);
+        TileCopy<BY, BY + 8, 64>::issue(s.yz, syz, g, x0, y0 - 4, L, tx, ty);
+    };
+    auto load_operands = [&](int k, VelOperands& u) {
+        u.zz = szz[dcell(g, xc, yc, k)];
+        if (active && k >= z0 && k < z1) {
+            if (qx >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.px[s] = P.p.psi[0][psi_index_x(g, s, qx, y, k)];
+            }
+            if (qy >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.py[s] = P.p.psi[1][psi_index_y(g, s, qy, x, k)];
+            }
+        }
+        const int z = k - 4;
+        if (active && z >= z0) {
+            const long long i = dcell(g, x, y, z);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) u.v[c] = P.p.f[c][i];
+            u.b[0] = ib[i]; u.b[1] = ib[i + 1]; u.b[2] = ib[i + g.px]; u.b[3] = ib[i + g.plane];
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            if (qz >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.pz[s] = P.p.psi[2][psi_index_z(g, s, qz, x, y)];
+            }
+        }
+    };
+
+    float q_xz[9], q_yz[9], q_zz[9];     // columns k-8..k (update plane z = k-4 in the middle)
+    float p_vx[5], p_vy[5], p_vz[5];     // CPML-filtered x+y sums of planes k-4..k
+#pragma unroll
+    for (int j = 0; j < 9; ++j) { q_xz[j] = 0.0f; q_yz[j] = 0.0f; q_zz[j] = 0.0f; }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { p_vx[j] = 0.0f; p_vy[j] = 0.0f; p_vz[j] = 0.0f; }
+
+    // Two operand sets used alternately (the loop is unrolled by two), so the
+    // prefetched registers are never copied while their loads are in flight.
+    VelOperands opa{}, opb{};
+    copy_plane(k0, k0);
+    load_operands(k0, opa);
+    copy_plane(k0 + 1, k0 + 1);
+    bool bad = false;
+    auto plane_step = [&](int k, const VelOperands& cur, VelOperands& nxt) {
+        const VelTiles<BY>& s = ring[(k - k0) % 3];
+        wait_vm<NDMA>();
+        __builtin_amdgcn_s_barrier();
+        if (k + 1 <= last) load_operands(k + 1, nxt);
+
+        shift(q_xz, s.xz[ty * 72 + tx + 4]);
+        shift(q_yz, s.yz[(ty + 4) * 64 + tx]);
+        shift(q_zz, cur.zz);
+        float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
+        if (active && k >= z0 && k < z1) {
+            float w[8];
+            // vx: D+x sxx (half), D-y sxy.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xx[ty * 72 + tx + 1 + j];
+            float d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * cur.px[0] + cx[3 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 0, qx, y, k)] = m;
+                d = cx[5 * 64 + tx] * d + m;
+            }
+            pvx += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + j) * 72 + tx + 4];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * cur.py[0] + cyt[0 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 0, qy, x, k)] = m;
+                d = cyt[2 * BY + ty] * d + m;
+            }
+            pvx += d;
+            // vy: D-x sxy, D+y syy (half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + 4) * 72 + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * cur.px[1] + cx[0 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 1, qx, y, k)] = m;
+                d = cx[2 * 64 + tx] * d + m;
+            }
+            pvy += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.yy[(ty + 1 + j) * 64 + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * cur.py[1] + cyt[3 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 1, qy, x, k)] = m;
+                d = cyt[5 * BY + ty] * d + m;
+            }
+            pvy += d;
+            // vz: D-x sxz, D-y syz.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xz[ty * 72 + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * cur.px[2] + cx[0 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 2, qx, y, k)] = m;
+                d = cx[2 * 64 + tx] * d + m;
+            }
+            pvz += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.yz[(ty + j) * 64 + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * cur.py[2] + cyt[0 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 2, qy, x, k)] = m;
+                d = cyt[2 * BY + ty] * d + m;
+            }
+            pvz += d;
+        }
+        shift(p_vx, pvx);
+        shift(p_vy, pvy);
+        shift(p_vz, pvz);
+
+        const int z = k - 4;
+        if (active && z >= z0) {
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            const int zt = z - z0;
+            const float c0[3] = {czt[0 * kMaxChunk + zt], czt[1 * kMaxChunk + zt], czt[2 * kMaxChunk + zt]};
+            const float c1[3] = {czt[3 * kMaxChunk + zt], czt[4 * kMaxChunk + zt], czt[5 * kMaxChunk + zt]};
+            const float hx = face_buoyancy(cur.b[0], cur.b[1]);
+            const float hy = face_buoyancy(cur.b[0], cur.b[2]);
+            const float hz = face_buoyancy(cur.b[0], cur.b[3]);
+            float w[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_xz[j];
+            float dz = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c0[1] * cur.pz[0] + c0[0] * dz;
+                P.p.psi[2][psi_index_z(g, 0, qz, x, y)] = m;
+                dz = c0[2] * dz + m;
+            }
+            const float div_x = p_vx[0] + dz;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_yz[j];
+            dz = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c0[1] * cur.pz[1] + c0[0] * dz;
+                P.p.psi[2][psi_index_z(g, 1, qz, x, y)] = m;
+                dz = c0[2] * dz + m;
+            }
+            const float div_y = p_vy[0] + dz;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_zz[j + 1];
+            dz = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c1[1] * cur.pz[2] + c1[0] * dz;
+                P.p.psi[2][psi_index_z(g, 2, qz, x, y)] = m;
+                dz = c1[2] * dz + m;
+            }
+            const float div_z = p_vz[0] + dz;
+            float vx = cur.v[0] + P.dt * hx * div_x;
+            float vy = cur.v[1] + P.dt * hy * div_y;
+            float vz = cur.v[2] + P.dt * hz * div_z;
+            if (P.src.active) {
+                const int r2 = source_r2d(P.src, x, y, z);
+                if (r2 >= 0) {
+                    const float value = P.src.scale * source_weight(P.src, r2) / P.src.norm;
+                    vx += value * P.src.dir[0] * hx;
+                    vy += value * P.src.dir[1] * hy;
+                    vz += value * P.src.dir[2] * hz;
+                }
+            }
+            const long long i = dcell(g, x, y, z);
+            P.p.f[0][i] = vx; P.p.f[1][i] = vy; P.p.f[2][i] = vz;
+            bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
+        }
+        // Last vector-memory instruction of the iteration: plane k+2 into the
+        // buffer every wave finished reading before this iteration's barrier
+        // (past the chunk end a clamped plane fills a buffer nobody reads).
+        copy_plane(k + 2, min(k + 2, last));
+    };
+    int k = k0;
+    for (; k + 1 <= last; k += 2) {
+        plane_step(k, opa, opb);
+        plane_step(k + 1, opb, opa);
+    }
+    if (k <= last) plane_step(k, opa, opb);
+    wait_vm<0>();
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+namespace {
+
+template <int BY>
+struct StrTiles {
+    float vx[(BY + 8) * 72];
+    float vy[(BY + 8) * 72];
+    float vz[(BY + 8) * 72];
+};
+
+template <int BY>
+constexpr int str_copies() {
+    return 3 * TileCopy<BY, BY + 8, 72>::COPIES;
+}
+
+struct StrOperands {
+    float px[3], py[3];  // x/y-slab CPML memories (slots 3-5) of the lead plane
+    float o[6];          // stresses at the update plane
+    float lam, mu;
+    float r[7];          // 1/mu at i, i+x, i+y, i+z, i+x+y, i+y+z, i+x+z
+    float pz[3];         // z-slab CPML memories at the update plane
+};
+
+}  // namespace
+
+// Fused stress half-step (stressKernel3D + explosion source epilogue).
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
+    __shared__ StrTiles<BY> ring[3];
+    __shared__ float cx[6 * 64];
+    __shared__ float cyt[6 * BY];
+    __shared__ float czt[6 * kMaxChunk];
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    const DevGrid& g = P.g;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = 4 + bz * M.zchunk;
+    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    const int k0 = z0 - 4, last = z1 + 3;
+    const bool active = x < g.gx - 4 && y < g.gy - 4;
+    const int qx = slab_q(x, g.gx, g.pml, false);
+    const int qy = slab_q(y, g.gy, g.pml, false);
+    const float invh = P.invh;
+    constexpr int NDMA = str_copies<BY>();
+
+    // CPML coefficient tables of this tile (read from LDS: no vector-memory
+    // waits inside the plane loop).
+    fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
+    __syncthreads();
+
+    auto copy_plane = [&](int slot, int L) {
+        StrTiles<BY>& s = ring[(slot - k0) % 3];
+        TileCopy<BY, BY + 8, 72>::issue(s.vx, P.p.f[0], g, x0 - 4, y0 - 4, L, tx, ty);
+        TileCopy<BY, BY + 8, 72>::issue(s.vy, P.p.f[1], g, x0 - 4, y0 - 4, L, tx, ty);
+        TileCopy<BY, BY + 8, 72>::issue(s.vz, P.p.f[2], g, x0 - 4, y0 - 4, L, tx, ty);
+    };
+    auto load_operands = [&](int k, StrOperands& u) {
+        if (active && k >= z0 && k < z1) {
+            if (qx >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.px[s] = P.p.psi[0][psi_index_x(g, 3 + s, qx, y, k)];
+            }
+            if (qy >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.py[s] = P.p.psi[1][psi_index_y(g, 3 + s, qy, x, k)];
+            }
+        }
+        const int z = k - 4;
+        if (active && z >= z0) {
+            const long long i = dcell(g, x, y, z);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) u.o[c] = P.p.f[3 + c][i];
+            u.lam = P.p.lambda[i];
+            u.mu = P.p.mu[i];
+            const float* r = P.p.inv_mu;
+            const long long sy = g.px, sz = g.plane;
+            u.r[0] = r[i]; u.r[1] = r[i + 1]; u.r[2] = r[i + sy]; u.r[3] = r[i + sz];
+            u.r[4] = r[i + 1 + sy]; u.r[5] = r[i + sy + sz]; u.r[6] = r[i + 1 + sz];
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            if (qz >= 0) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) u.pz[s] = P.p.psi[2][psi_index_z(g, 3 + s, qz, x, y)];
+            }
+        }
+    };
+
+    float q_vx[9], q_vy[9], q_vz[9];
+    float p00[5], p11[5], p01[5], p20[5], p21[5];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) { q_vx[j] = 0.0f; q_vy[j] = 0.0f; q_vz[j] = 0.0f; }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { p00[j] = 0.0f; p11[j] = 0.0f; p01[j] = 0.0f; p20[j] = 0.0f; p21[j] = 0.0f; }
+
+    // Two operand sets used alternately (the loop is unrolled by two), so the
+    // prefetched registers are never copied while their loads are in flight.
+    StrOperands opa{}, opb{};
+    copy_plane(k0, k0);
+    load_operands(k0, opa);
+    copy_plane(k0 + 1, k0 + 1);
+    bool bad = false;
+    auto plane_step = [&](int k, const StrOperands& cur, StrOperands& nxt) {
+        const StrTiles<BY>& s = ring[(k - k0) % 3];
+        wait_vm<NDMA>();
+        __builtin_amdgcn_s_barrier();
+        if (k + 1 <= last) load_operands(k + 1, nxt);
+        const int c = (ty + 4) * 72 + tx + 4;
+        shift(q_vx, s.vx[c]);
+        shift(q_vy, s.vy[c]);
+        shift(q_vz, s.vz[c]);
+        float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
+        if (active && k >= z0 && k < z1) {
+            float w[8];
+            // vx: D-x (slot 3, integer), D+y (slot 3, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 4) * 72 + tx + j];
+            d00 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * cur.px[0] + cx[0 * 64 + tx] * d00;
+                P.p.psi[0][psi_index_x(g, 3, qx, y, k)] = m;
+                d00 = cx[2 * 64 + tx] * d00 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 1 + j) * 72 + tx + 4];
+            float d01 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * cur.py[0] + cyt[3 * BY + ty] * d01;
+                P.p.psi[1][psi_index_y(g, 3, qy, x, k)] = m;
+                d01 = cyt[5 * BY + ty] * d01 + m;
+            }
+            // vy: D+x (slot 4, half), D-y (slot 4, integer).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + 4) * 72 + tx + 1 + j];
+            float d10 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * cur.px[1] + cx[3 * 64 + tx] * d10;
+                P.p.psi[0][psi_index_x(g, 4, qx, y, k)] = m;
+                d10 = cx[5 * 64 + tx] * d10 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + j) * 72 + tx + 4];
+            d11 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * cur.py[1] + cyt[0 * BY + ty] * d11;
+                P.p.psi[1][psi_index_y(g, 4, qy, x, k)] = m;
+                d11 = cyt[2 * BY + ty] * d11 + m;
+            }
+            s01 = d01 + d10;
+            // vz: D+x (slot 5, half), D+y (slot 5, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 4) * 72 + tx + 1 + j];
+            d20 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * cur.px[2] + cx[3 * 64 + tx] * d20;
+                P.p.psi[0][psi_index_x(g, 5, qx, y, k)] = m;
+                d20 = cx[5 * 64 + tx] * d20 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 1 + j) * 72 + tx + 4];
+            d21 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * cur.py[2] + cyt[3 * BY + ty] * d21;
+                P.p.psi[1][psi_index_y(g, 5, qy, x, k)] = m;
+                d21 = cyt[5 * BY + ty] * d21 + m;
+            }
+        }
+        shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
+
+        const int z = k - 4;
+        if (active && z >= z0) {
+            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+            const int zt = z - z0;
+            const float c0[3] = {czt[0 * kMaxChunk + zt], czt[1 * kMaxChunk + zt], czt[2 * kMaxChunk + zt]};
+            const float c1[3] = {czt[3 * kMaxChunk + zt], czt[4 * kMaxChunk + zt], czt[5 * kMaxChunk + zt]};
+            float w[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vx[j + 1];
+            float d02 = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c1[1] * cur.pz[0] + c1[0] * d02;
+                P.p.psi[2][psi_index_z(g, 3, qz, x, y)] = m;
+                d02 = c1[2] * d02 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vy[j + 1];
+            float d12 = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c1[1] * cur.pz[1] + c1[0] * d12;
+                P.p.psi[2][psi_index_z(g, 4, qz, x, y)] = m;
+                d12 = c1[2] * d12 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = q_vz[j];
+            float d22 = stencil8(w, invh);
+            if (qz >= 0) {
+                const float m = c0[1] * cur.pz[2] + c0[0] * d22;
+                P.p.psi[2][psi_index_z(g, 5, qz, x, y)] = m;
+                d22 = c0[2] * d22 + m;
+            }
+            const bool surface = g.free_surface && z == 4;
+            const float lam = cur.lam, mu = cur.mu;
+            const float e00 = p00[0], e11 = p11[0];
+            if (surface) d22 = -lam / (lam + 2 * mu) * (e00 + e11);
+            const float trace = e00 + e11 + d22;
+            float sxx = cur.o[0] + P.dt * (lam * trace + 2 * mu * e00);
+            float syy = cur.o[1] + P.dt * (lam * trace + 2 * mu * e11);
+            float szz = surface ? 0.0f : cur.o[2] + P.dt * (lam * trace + 2 * mu * d22);
+            const float sxy = cur.o[3] + P.dt * shear_from_inverse(cur.r[0], cur.r[1], cur.r[2], cur.r[4]) * p01[0];
+            const float syz = cur.o[4] + P.dt * shear_from_inverse(cur.r[0], cur.r[2], cur.r[3], cur.r[5]) * (d12 + p21[0]);
+            const float sxz = cur.o[5] + P.dt * shear_from_inverse(cur.r[0], cur.r[1], cur.r[3], cur.r[6]) * (d02 + p20[0]);
+            if (P.src.active) {
+                const int r2 = source_r2d(P.src, x, y, z);
+                if (r2 >= 0) {
+                    const float value = P.src.scale * source_weight(P.src, r2) / P.src.norm;
+                    sxx += value; syy += value; szz += value;
+                }
+            }
+            const long long i = dcell(g, x, y, z);
+            P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
+            P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
+            bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) ||
+                   !isfinite(sxz);
+        }
+        copy_plane(k + 2, min(k + 2, last));
+    };
+    int k = k0;
+    for (; k + 1 <= last; k += 2) {
+        plane_step(k, opa, opb);
+        plane_step(k + 1, opb, opa);
+    }
+    if (k <= last) plane_step(k, opa, opb);
+    wait_vm<0>();
+    if (bad) atomicOr(P.p.flags, 1u);
+}
+
+hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {
+    if (zchunk > kMaxChunk) return hipErrorInvalidValue;
+    TileMap m;
+    m.nbx = (P.g.gx - 8 + 63) / 64;
+    m.nby = (P.g.gy - 8 + tile_y - 1) / tile_y;
+    m.zchunk = zchunk;
+    m.nbz = (P.g.gz - 8 + zchunk - 1) / zchunk;
+    m.nblocks = m.nbx * m.nby * m.nbz;
+    const int launched = (m.nblocks + 7) / 8 * 8;
+    m.per_xcd = launched / 8;
+    const dim3 grid(launched);
+    switch (tile_y) {
+    case 4:
+        if (velocity) hipLaunchKernelGGL(velocity_dma<4>, grid, dim3(64, 4), 0, s, P, m);
+        else hipLaunchKernelGGL(stress_dma<4>, grid, dim3(64, 4), 0, s, P, m);
+        break;
+    case 8:
+        if (velocity) hipLaunchKernelGGL(velocity_dma<8>, grid, dim3(64, 8), 0, s, P, m);
+        else hipLaunchKernelGGL(stress_dma<8>, grid, dim3(64, 8), 0, s, P, m);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace seis3d

@@ -124,13 +124,13 @@ __global__ __launch_bounds__(256) void stress_direct(DevStep P) {
     const bool surface = g.free_surface && z == 4;
     if (surface) d[2][2] = -lam / (lam + 2 * mu) * (d[0][0] + d[1][1]);
     const float trace = d[0][0] + d[1][1] + d[2][2];
-    const float* m = P.p.mu;
+    const float* r = P.p.inv_mu;
     float sxx = P.p.f[3][i] + P.dt * (lam * trace + 2 * mu * d[0][0]);
     float syy = P.p.f[4][i] + P.dt * (lam * trace + 2 * mu * d[1][1]);
     float szz = surface ? 0.0f : P.p.f[5][i] + P.dt * (lam * trace + 2 * mu * d[2][2]);
-    const float sxy = P.p.f[6][i] + P.dt * shear_harmonic(m[i], m[i + st[0]], m[i + st[1]], m[i + st[0] + st[1]]) * (d[0][1] + d[1][0]);
-    const float syz = P.p.f[7][i] + P.dt * shear_harmonic(m[i], m[i + st[1]], m[i + st[2]], m[i + st[1] + st[2]]) * (d[1][2] + d[2][1]);
-    const float sxz = P.p.f[8][i] + P.dt * shear_harmonic(m[i], m[i + st[0]], m[i + st[2]], m[i + st[0] + st[2]]) * (d[0][2] + d[2][0]);
+    const float sxy = P.p.f[6][i] + P.dt * shear_from_inverse(r[i], r[i + st[0]], r[i + st[1]], r[i + st[0] + st[1]]) * (d[0][1] + d[1][0]);
+    const float syz = P.p.f[7][i] + P.dt * shear_from_inverse(r[i], r[i + st[1]], r[i + st[2]], r[i + st[1] + st[2]]) * (d[1][2] + d[2][1]);
+    const float sxz = P.p.f[8][i] + P.dt * shear_from_inverse(r[i], r[i + st[0]], r[i + st[2]], r[i + st[0] + st[2]]) * (d[0][2] + d[2][0]);
     if (P.src.active) {
         const int r2 = source_r2(P.src, x, y, z);
         if (r2 >= 0) {
@@ -147,30 +147,6 @@ __global__ __launch_bounds__(256) void stress_direct(DevStep P) {
 // ---------------------------------------------------------------------------
 // Stream (2.5D) family
 // ---------------------------------------------------------------------------
-
-constexpr int TX = 64;     // one wavefront across x
-constexpr int HX = TX + 8; // x extent with a 4-cell halo each side
-
-struct TileMap {
-    int nbx, nby, nbz;     // tiles per axis
-    int nblocks;           // nbx * nby * nbz
-    int per_xcd;           // launched blocks / 8
-    int zchunk;            // planes per tile
-};
-
-// XCD-aware tile assignment: the hardware deals workgroup b to XCD b % 8, so
-// logical tile (b % 8) * per_xcd + b / 8 gives each XCD a contiguous run of
-// x-fastest tiles (neighbours share halos in that XCD's L2).
-__device__ __forceinline__ bool tile_of(const TileMap& m, int& bx, int& by, int& bz) {
-    const int b = int(blockIdx.x);
-    const int logical = (b & 7) * m.per_xcd + (b >> 3);
-    if (logical >= m.nblocks) return false;
-    bx = logical % m.nbx;
-    const int t = logical / m.nbx;
-    by = t % m.nby;
-    bz = t / m.nby;
-    return true;
-}
 
 // Cooperative load of a ROWS x COLS window of one plane into LDS (row
 // stride COLS).  Coordinates are clamped into the padded grid: clamped
@@ -190,12 +166,6 @@ __device__ __forceinline__ void stage(float* __restrict__ lds, const float* __re
     }
 }
 
-template <int N>
-__device__ __forceinline__ void shift(float (&q)[N], float lead) {
-#pragma unroll
-    for (int j = 0; j < N - 1; ++j) q[j] = q[j + 1];
-    q[N - 1] = lead;
-}
 
 // Fused velocity half-step, 2.5D.  Staged at the lead plane L = z+4:
 // sxx (x halo), sxy (x+y halo), syy (y halo), sxz (x halo), syz (y halo);
@@ -472,10 +442,11 @@ __global__ __launch_bounds__(TX * BY) void stress_stream(DevStep P, TileMap M) {
             float sxx = P.p.f[3][i] + P.dt * (lam * trace + 2 * mu * d00);
             float syy = P.p.f[4][i] + P.dt * (lam * trace + 2 * mu * d11);
             float szz = surface ? 0.0f : P.p.f[5][i] + P.dt * (lam * trace + 2 * mu * d22);
-            const float mx = m[i + sx], my = m[i + sy], mz = m[i + sz];
-            const float sxy = P.p.f[6][i] + P.dt * shear_harmonic(mu, mx, my, m[i + sx + sy]) * p01[0];
-            const float syz = P.p.f[7][i] + P.dt * shear_harmonic(mu, my, mz, m[i + sy + sz]) * (d12 + p21[0]);
-            const float sxz = P.p.f[8][i] + P.dt * shear_harmonic(mu, mx, mz, m[i + sx + sz]) * (d02 + p20[0]);
+            const float* r = P.p.inv_mu;
+            const float r0 = r[i], rx = r[i + sx], ry = r[i + sy], rz = r[i + sz];
+            const float sxy = P.p.f[6][i] + P.dt * shear_from_inverse(r0, rx, ry, r[i + sx + sy]) * p01[0];
+            const float syz = P.p.f[7][i] + P.dt * shear_from_inverse(r0, ry, rz, r[i + sy + sz]) * (d12 + p21[0]);
+            const float sxz = P.p.f[8][i] + P.dt * shear_from_inverse(r0, rx, rz, r[i + sx + sz]) * (d02 + p20[0]);
             if (P.src.active) {
                 const int r2 = source_r2(P.src, x, y, z);
                 if (r2 >= 0) {
@@ -656,6 +627,19 @@ __global__ void extend_materials(DevStep P, MaterialParams mp) {
     P.p.lambda[i] = P.p.lambda[j];
 }
 
+// 1/mu over the padded grid: the reciprocals of shearMu3D
+// (cuda/Cuda_Seis3D.cu:275-278), computed once per model instead of four
+// times per cell and step.
+__global__ void reciprocal_mu(DevStep P) {
+    const DevGrid& g = P.g;
+    const long long cells = static_cast<long long>(g.gx) * g.gy * g.gz;
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= cells) return;
+    const int x = int(k % g.gx), y = int((k / g.gx) % g.gy), z = int(k / (static_cast<long long>(g.gx) * g.gy));
+    const long long i = cell(g, x, y, z);
+    P.p.inv_mu[i] = 1 / P.p.mu[i];
+}
+
 // Packed slice of one collocated component over the display domain
 // (extractKernel3D slice part, cuda/Cuda_Seis3D.cu:370-394, without GL).
 __global__ void extract_slice(DevStep P, SliceParams sp, float* __restrict__ out) {
@@ -706,6 +690,9 @@ static hipError_t stream_launch(bool velocity, const DevStep& P, int tile_y, int
 }
 
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s) {
+    if (k.family == 4) return launch_dma(true, P, k.tile_y, k.z_chunk, s);
+    if (k.family == 3) return launch_column(true, P, k.tile_y, k.z_chunk, s);
+    if (k.family == 2) return launch_pipe(true, P, k.tile_y, k.z_chunk, s);
     if (k.family == 1) return stream_launch(true, P, k.tile_y, k.z_chunk, s);
     const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
     hipLaunchKernelGGL(velocity_direct, grid, block, 0, s, P);
@@ -713,6 +700,9 @@ hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t 
 }
 
 hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s) {
+    if (k.family == 4) return launch_dma(false, P, k.tile_y, k.z_chunk, s);
+    if (k.family == 3) return launch_column(false, P, k.tile_y, k.z_chunk, s);
+    if (k.family == 2) return launch_pipe(false, P, k.tile_y, k.z_chunk, s);
     if (k.family == 1) return stream_launch(false, P, k.tile_y, k.z_chunk, s);
     const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
     hipLaunchKernelGGL(stress_direct, grid, block, 0, s, P);
@@ -758,6 +748,12 @@ hipError_t launch_generate_materials(const DevStep& P, const MaterialParams& mp,
 hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s) {
     const long long cells = static_cast<long long>(P.g.gx) * P.g.gy * P.g.gz;
     hipLaunchKernelGGL(extend_materials, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, s, P, mp);
+    return hipGetLastError();
+}
+
+hipError_t launch_reciprocal_mu(const DevStep& P, hipStream_t s) {
+    const long long cells = static_cast<long long>(P.g.gx) * P.g.gy * P.g.gz;
+    hipLaunchKernelGGL(reciprocal_mu, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, s, P);
     return hipGetLastError();
 }
 

@@ -9,8 +9,8 @@ namespace seis3d {
 
 // Kernel family and tile shape of the two fused half-steps.
 struct KernelConfig {
-    int family = 1;    // 0 direct, 1 stream (2.5D)
-    int tile_y = 8;    // rows per workgroup (stream): 4, 8 or 16
+    int family = 2;    // 0 direct, 1 stream (2.5D), 2 pipelined stream (2.5D), 3 column march, 4 LDS-DMA stream (2.5D)
+    int tile_y = 8;    // rows per workgroup: 4, 8 or 16 (family 1), 4 or 8 (family 2)
     int z_chunk = 32;  // planes per workgroup (stream)
 };
 
@@ -32,6 +32,12 @@ struct SliceParams {
 };
 
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s);
+// Software-pipelined 2.5D family (seis3d_pipe.hip), tile_y 4 or 8.
+hipError_t launch_pipe(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
+// Column-marching family (seis3d_column.hip), tile_y 1, 2, 4 or 8.
+hipError_t launch_column(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
+// LDS-DMA staged 2.5D family (seis3d_dma.hip), tile_y 4 or 8.
+hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
 hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s);
 hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s);
 hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s);
@@ -39,6 +45,7 @@ hipError_t launch_reflecting_walls(const DevStep& P, bool velocities, hipStream_
 hipError_t launch_receivers(const DevStep& P, const int* rec, int count, float* out, hipStream_t s);
 hipError_t launch_generate_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);
 hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);
+hipError_t launch_reciprocal_mu(const DevStep& P, hipStream_t s);
 hipError_t launch_extract_slice(const DevStep& P, const SliceParams& sp, float* out, hipStream_t s);
 
 }  // namespace seis3d

@@ -50,7 +50,7 @@ CASES = {
         dict(x=12, y=10, z=11, kind=S.EXPLOSION), None, 100, 2),
 }
 
-TILES = [(0, 8, 32), (1, 4, 7), (1, 8, 16), (1, 16, 5), (1, 8, 1000)]
+TILES = [(0, 8, 32), (1, 4, 7), (1, 8, 16), (1, 16, 5), (2, 4, 7), (2, 8, 16), (2, 8, 1000), (3, 1, 6), (3, 4, 16), (3, 8, 1000), (4, 4, 5), (4, 8, 16), (4, 8, 128)]
 
 
 def receivers_for(c):
@@ -136,7 +136,7 @@ def test_stream_equals_direct_bitwise(gpu):
     model = W.layered_model(c.nx, c.ny, c.nz, seed=9)
     src = W.config2_source(c)
     out = []
-    for tiles in [(0, 8, 32), (1, 8, 13), (1, 16, 64)]:
+    for tiles in [(0, 8, 32), (1, 8, 13), (1, 16, 64), (2, 4, 9), (2, 8, 40), (3, 2, 11), (3, 4, 40), (4, 4, 7), (4, 8, 20)]:
         g = S.GPUSimData3D(c)
         g.set_tiles(*tiles)
         g.upload_model(*model)
@@ -144,8 +144,8 @@ def test_stream_equals_direct_bitwise(gpu):
         out.append([g.download_field(f) for f in range(9)])
         g.release()
     for f in range(9):
-        assert np.array_equal(out[0][f], out[1][f]), f
-        assert np.array_equal(out[0][f], out[2][f]), f
+        for k in range(1, len(out)):
+            assert np.array_equal(out[0][f], out[k][f]), (f, k)
 
 
 def test_config2_full_size_parity(gpu):
@@ -156,7 +156,7 @@ def test_config2_full_size_parity(gpu):
     rec = W.config2_receivers(c)
     steps, stride = 30, 2
     log("config2 full size: GPU run")
-    g, gather = run_gpu(c, src, model, steps, stride, rec, (1, 8, 40))
+    g, gather = run_gpu(c, src, model, steps, stride, rec, (2, 8, 40))
     log("config2 full size: oracle run")
     O.set_threads(16)
     o, g32 = run_oracle(c, src, model, steps, stride, 32, rec)
