@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", default="300x50x300")
+    p.add_argument("--workload", choices=["step", "batch"], default="step",
+                   help="step: the fused 3D step (default line); batch: 16 uneven shots, graph vs plain loop")
+    p.add_argument("--shots", type=int, default=16)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="instead of the bench, run 5 plain device copies of MIB MiB (counter calibration)")
     return p.parse_args()
@@ -58,6 +61,101 @@ def copy_probe(mib: int) -> None:
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+
+
+def profiled_traffic(kernel: str, grid: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload
+    (profiles/<family>_kernels_<grid>.md, line 'traffic_bytes_per_launch: name=bytes ...')."""
+    for path in sorted((ROOT / "profiles").glob(f"*_{grid}.md")):
+        for line in path.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                for item in line.split(":", 1)[1].split():
+                    name, _, value = item.partition("=")
+                    if name == kernel:
+                        return int(value), path.name
+    return None, None
+
+
+def batch_bench(args) -> None:
+    """16 shots on uneven models (128^3 .. 256x128x256), 800 steps, 100 receivers each, stride 2.
+
+    plain: the reference's shot loop, one shot after another, runStep3D per step and
+    sampleReceivers3D (launch + copy + stream sync) every stride (src/Seis3D_Sim.cpp:120-153).
+    batched: every shot on its own stream, all steps captured in one HIP graph, one launch,
+    device-side gathers.  Model upload, capture and instantiation are setup, outside both timings.
+    """
+    import numpy as np
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    from cmakepro_cuda_seissim_v2_amd import workloads as W
+
+    steps, stride = args.steps, 2
+    shots = W.batch_shots(args.shots)
+    sessions = []
+    for ctx, model, src, rec in shots:
+        g = S.GPUSimData3D(ctx)
+        g.upload_model(*model)
+        g.set_receivers(rec)
+        sessions.append(g)
+    tuned = {}
+    for (ctx, model, src, rec), g in zip(shots, sessions):
+        key = (ctx.gx, ctx.gy, ctx.gz)
+        tuned[key] = g.autotune(src, steps=3)
+    n_upd = sum((c.gx - 8) * (c.gy - 8) * (c.gz - 8) for c, _, _, _ in shots)
+    print(f"bench: batch of {len(shots)} shots, {n_upd / 1e6:.1f} M updated cells", file=sys.stderr, flush=True)
+
+    # Plain per-shot loop.
+    for g in sessions:
+        g.reset()
+        g.synchronize()
+    t0 = time.perf_counter()
+    plain = []
+    for (ctx, model, src, rec), g in zip(shots, sessions):
+        samples = []
+        for n in range(steps):
+            g.step(src, n)
+            if (n + 1) % stride == 0:
+                samples.append(g.sample_receivers())
+        g.synchronize()
+        plain.append(np.stack(samples))
+    t_plain = time.perf_counter() - t0
+    print(f"bench: plain loop {t_plain:.3f} s", file=sys.stderr, flush=True)
+
+    # Batched graph.
+    for g in sessions:
+        g.reset()
+        g.gather_begin(steps // stride, stride)
+    batch = S.Batch(sessions, [s[2] for s in shots], steps)
+    for g in sessions:
+        g.synchronize()
+    t0 = time.perf_counter()
+    batch.launch()
+    batch.synchronize()
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(g.gather_read(), p) for g, p in zip(sessions, plain))
+    print(f"bench: batched graph {t_batch:.3f} s, gathers identical to plain: {same}", file=sys.stderr, flush=True)
+    batch.close()
+    for g in sessions:
+        g.release()
+    out = {
+        "metric": "shots/s, 16-shot batched 3D O(8) forward (one hipGraph launch) vs the plain per-shot loop",
+        "value": round(len(shots) / t_batch, 4),
+        "unit": "shots/s",
+        "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(t_batch / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded gradient + Gaussian random-field models)",
+        "config": {
+            "workload": f"{len(shots)} shots, nx,nz in 128..256 step 16, ny 128, 20-cell CPML, free top, "
+                        f"{steps} steps, 15 Hz force, 100 receivers stride {stride}",
+            "updated_cells_total": n_upd,
+            "plain_shots_per_s": round(len(shots) / t_plain, 4),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "gathers_bit_identical": bool(same),
+            "gcell_updates_per_s_batched": round(n_upd * steps / t_batch / 1e9, 3),
+            "autotuned": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in tuned.items()},
+        },
+    }
+    print(json.dumps(out), flush=True)
 
 
 def cpu_info():
@@ -106,6 +204,11 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "batch":
+        if args.steps == 1000:
+            args.steps = 800
+        batch_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -191,11 +294,16 @@ def main():
             "unit": "GB/s",
             "frac": round(dominant[3] / HBM_PEAK_GBS, 4),
             "traffic": None,
+            "traffic_source": None,
             "algorithmic_bytes_per_launch": dominant[2],
             "avg_launch_us": round(dominant[1], 3),
         },
         "cpu_baseline": None,
     }
+    tb, src_file = profiled_traffic(out["roofline"]["kernel"], args.grid)
+    if tb:
+        out["roofline"]["traffic"] = tb
+        out["roofline"]["traffic_source"] = f"profiles/{src_file} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ctx, model, src, rec, args.cpu_steps)

@@ -72,10 +72,28 @@ def build_oracle() -> Path:
     return out
 
 
+CPP_TESTS = ["seis3d_boundary_regression"]
+
+
+def build_cpp_tests() -> list[Path]:
+    """Compile the C++ test programs (tests/cpp) against include/Seis3D.hpp and libseis3d.so."""
+    src_dir = REPO_DIR / "tests" / "cpp"
+    out_dir = src_dir / "bin"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    outs = []
+    for name in CPP_TESTS:
+        out = out_dir / name
+        _run([CXX, "-O2", "-std=c++17", "-Wall", f"-I{REPO_DIR / 'include'}", str(src_dir / f"{name}.cpp"),
+              "-o", str(out), f"-L{LIB_DIR}", "-lseis3d", f"-Wl,-rpath,$ORIGIN/../../../cmakepro_cuda_seissim_v2_amd/lib"])
+        outs.append(out)
+    return outs
+
+
 def build_all() -> None:
     for name in LIBRARIES:
         build_library(name)
     build_oracle()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

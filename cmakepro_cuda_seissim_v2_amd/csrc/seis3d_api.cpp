@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1050,5 +1053,182 @@ int seis3d_step_bytes(const seis3d_ctx* ctx, uint64_t* total, uint64_t* velocity
 const char* seis3d_last_error(void) { return g_error.c_str(); }
 
 const char* seis3d_version(void) { return "seiswave-mi355x 0.1.0 (gfx950, HIP)"; }
+
+int seis3d_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner and multi-shot batches
+// ---------------------------------------------------------------------------
+
+namespace {
+
+using TileKey = std::tuple<int, int, int, int, int, int>;
+std::mutex g_tune_mutex;
+std::map<TileKey, seis3d::KernelConfig> g_tuned;
+
+TileKey tile_key(const seis3d_ctx& c) { return {c.gx, c.gy, c.gz, c.pml, c.free_surface, c.reflecting}; }
+
+// Candidate shapes: the direct family and the LDS-DMA family over tile
+// heights and z-chunk lengths that keep at least a few hundred workgroups.
+std::vector<seis3d::KernelConfig> tile_candidates(const seis3d_ctx& c) {
+    std::vector<seis3d::KernelConfig> out;
+    out.push_back({0, 8, 32});
+    const int uz = c.gz - 8;
+    for (int ty : {4, 8})
+        for (int zc : {8, 12, 16, 24, 32})
+            if (zc <= uz && zc <= 128) out.push_back({4, ty, zc});
+    return out;
+}
+
+}  // namespace
+
+struct seis3d_batch {
+    std::vector<seis3d_t*> sessions;
+    hipStream_t origin = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;
+    int64_t steps = 0;
+
+    void release() noexcept {
+        if (origin) (void)hipStreamSynchronize(origin);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (origin) (void)hipStreamDestroy(origin);
+        exec = nullptr; graph = nullptr; origin = nullptr;
+        events.clear();
+    }
+};
+
+extern "C" {
+
+int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* family, int* tile_y, int* z_chunk,
+                    double* best_ms) {
+    return guarded([&] {
+        require_session(t);
+        if (!src || steps < 1 || steps > 1000) invalid("Seis3D: invalid autotune request.");
+        check_source(t->c, *src);
+        const TileKey key = tile_key(t->c);
+        seis3d::KernelConfig best{};
+        double best_t = -1;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) { best = it->second; best_t = 0; }
+        }
+        if (best_t < 0) {
+            // Time every candidate on a scratch session of the same grid, so the
+            // caller's wavefield and model are untouched.
+            seis3d_t* scratch = nullptr;
+            const int rc = seis3d_create(&t->c, &scratch);
+            if (rc != SEIS3D_OK) fail(rc, g_error);
+            try {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                check(hipEventCreate(&e0), "create event");
+                check(hipEventCreate(&e1), "create event");
+                for (const auto& cand : tile_candidates(t->c)) {
+                    scratch->kernels = cand;
+                    enqueue_step(scratch, *src, 0);   // warm-up
+                    check(hipEventRecord(e0, scratch->stream), "record event");
+                    for (int k = 0; k < steps; ++k) enqueue_step(scratch, *src, std::uint64_t(k + 1));
+                    check(hipEventRecord(e1, scratch->stream), "record event");
+                    check(hipEventSynchronize(e1), "autotune");
+                    float ms = 0;
+                    check(hipEventElapsedTime(&ms, e0, e1), "read event");
+                    if (best_t < 0 || ms < best_t) { best_t = ms; best = cand; }
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            } catch (...) {
+                seis3d_destroy(scratch);
+                throw;
+            }
+            seis3d_destroy(scratch);
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            g_tuned[key] = best;
+        }
+        t->kernels = best;
+        if (family) *family = best.family;
+        if (tile_y) *tile_y = best.tile_y;
+        if (z_chunk) *z_chunk = best.z_chunk;
+        if (best_ms) *best_ms = best_t / steps;
+    });
+}
+
+int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources, int count, int64_t n_steps,
+                        seis3d_batch_t** out) {
+    return guarded([&] {
+        if (!sessions || !sources || !out || count < 1 || count > 256 || n_steps < 1 || n_steps > 1000000)
+            invalid("Seis3D: invalid batch request.");
+        for (int s = 0; s < count; ++s) {
+            require_session(sessions[s]);
+            check_source(sessions[s]->c, sources[s]);
+            for (int r = 0; r < s; ++r)
+                if (sessions[r] == sessions[s]) invalid("Seis3D: a session appears twice in one batch.");
+        }
+        auto* b = new seis3d_batch();
+        b->sessions.assign(sessions, sessions + count);
+        b->steps = n_steps;
+        try {
+            check(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking), "create stream");
+            b->events.resize(std::size_t(count) + 1, nullptr);
+            for (auto& e : b->events) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+            // Every session's stream joins one capture: fork from the origin stream,
+            // enqueue all steps of every shot on its own stream, join back.
+            for (auto* t : b->sessions) check(hipStreamSynchronize(t->stream), "batch capture");
+            check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "begin capture");
+            check(hipEventRecord(b->events[0], b->origin), "fork capture");
+            for (int s = 0; s < count; ++s) {
+                seis3d_t* t = b->sessions[s];
+                check(hipStreamWaitEvent(t->stream, b->events[0], 0), "fork capture");
+                for (int64_t k = 0; k < n_steps; ++k) {
+                    enqueue_step(t, sources[s], std::uint64_t(k));
+                    maybe_sample(t, std::uint64_t(k));
+                }
+                check(hipEventRecord(b->events[s + 1], t->stream), "join capture");
+                check(hipStreamWaitEvent(b->origin, b->events[s + 1], 0), "join capture");
+            }
+            check(hipStreamEndCapture(b->origin, &b->graph), "end capture");
+            check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate graph");
+        } catch (...) {
+            if (b->origin) {
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(b->origin, &partial);
+                if (partial) (void)hipGraphDestroy(partial);
+            }
+            b->release();
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int seis3d_batch_launch(seis3d_batch_t* b) {
+    return guarded([&] {
+        if (!b || !b->exec) fail(SEIS3D_ELIFECYCLE, "Seis3D: batch not created.");
+        check(hipGraphLaunch(b->exec, b->origin), "launch batch");
+    });
+}
+
+int seis3d_batch_synchronize(seis3d_batch_t* b) {
+    return guarded([&] {
+        if (!b || !b->origin) fail(SEIS3D_ELIFECYCLE, "Seis3D: batch not created.");
+        check(hipStreamSynchronize(b->origin), "finish batch");
+    });
+}
+
+void seis3d_batch_destroy(seis3d_batch_t* b) {
+    if (!b) return;
+    b->release();
+    delete b;
+}
 
 }  // extern "C"

@@ -155,6 +155,13 @@ def library() -> C.CDLL:
                                             P(C.c_uint64), P(C.c_uint64)]),
             "seis3d_last_error": (C.c_char_p, []),
             "seis3d_version": (C.c_char_p, []),
+            "seis3d_device_count": (C.c_int, []),
+            "seis3d_autotune": (C.c_int, [vp, P(Source), C.c_int, P(C.c_int), P(C.c_int), P(C.c_int),
+                                          P(C.c_double)]),
+            "seis3d_batch_create": (C.c_int, [P(vp), P(Source), C.c_int, C.c_int64, P(vp)]),
+            "seis3d_batch_launch": (C.c_int, [vp]),
+            "seis3d_batch_synchronize": (C.c_int, [vp]),
+            "seis3d_batch_destroy": (None, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -455,10 +462,55 @@ class GPUSimData3D:
     def set_tiles(self, family: int, tile_y: int = 8, z_chunk: int = 32) -> None:
         _check(library().seis3d_set_tiles(self._h, family, tile_y, z_chunk))
 
+    def autotune(self, src: Source, steps: int = 5) -> dict:
+        """Pick the fastest kernel shape for this grid (cached per grid for the process)."""
+        f, y, z = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        _check(library().seis3d_autotune(self._h, C.byref(src), int(steps), C.byref(f), C.byref(y), C.byref(z),
+                                         C.byref(ms)))
+        return {"family": f.value, "tile_y": y.value, "z_chunk": z.value, "ms_per_step": ms.value}
+
     def get_tiles(self) -> tuple[int, int, int]:
         f, y, z = C.c_int(), C.c_int(), C.c_int()
         _check(library().seis3d_get_tiles(self._h, C.byref(f), C.byref(y), C.byref(z)))
         return f.value, y.value, z.value
+
+
+class Batch:
+    """Several shots captured into one HIP graph (one launch runs every shot's steps).
+
+    Each session needs its model, receivers and an open gather (gather_begin) before the batch is
+    created; after launch() + synchronize() every session's gather holds its shot record.
+    """
+
+    def __init__(self, sessions, sources, n_steps: int):
+        self.sessions = list(sessions)
+        n = len(self.sessions)
+        hs = (C.c_void_p * n)(*[g.handle.value for g in self.sessions])
+        srcs = (Source * n)(*sources)
+        self._b = C.c_void_p()
+        _check(library().seis3d_batch_create(hs, srcs, n, int(n_steps), C.byref(self._b)))
+
+    def launch(self) -> None:
+        _check(library().seis3d_batch_launch(self._b))
+
+    def synchronize(self) -> None:
+        _check(library().seis3d_batch_synchronize(self._b))
+
+    def close(self) -> None:
+        if self._b.value:
+            library().seis3d_batch_destroy(self._b)
+        self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return int(library().seis3d_device_count())
 
 
 # Reference-named free functions (cuda/Cuda_Seis3D.cuh:6-42).

@@ -78,3 +78,32 @@ def config2_receivers(ctx: S.Ctx):
     """The default 50 x 3 surface spread of fitSurvey3D/configureReceivers3D."""
     q = S.fitSurvey3D(ctx, S.default_survey())
     return S.configureReceivers3D(ctx, q)
+
+
+def batch_shots(count: int = 16, seed: int = BASE_SEED, sizes=None, corr: float = 20.0, receivers: int = 100,
+                pml: int = 20, frequency: float = 15.0):
+    """Uneven multi-shot workload: per shot (context, (vp, vs, rho), source, receivers).
+
+    nx, nz drawn from 128..256 in steps of 16 and ny = 128 unless `sizes` gives (nx, ny, nz) per shot;
+    smooth vp 2000 -> 4000 m/s depth gradient times (1 + 0.1 g) with a seeded Gaussian random field
+    g of correlation length `corr` cells; a seeded 15 Hz vertical force; a `receivers`-long surface
+    line along x through the model's centre.
+    """
+    rng = np.random.default_rng(seed)
+    shots = []
+    for k in range(count):
+        if sizes is not None:
+            nx, ny, nz = sizes[k]
+        else:
+            nx = int(rng.choice(np.arange(128, 257, 16)))
+            nz = int(rng.choice(np.arange(128, 257, 16)))
+            ny = 128
+        vp, vs, rho = gradient_random_model(nx, ny, nz, seed=seed + 1 + k, corr=corr)
+        ctx = imported_context(nx, ny, nz, max_vp=float(vp.max()), pml=pml, frequency=frequency)
+        sx = int(rng.integers(nx // 4, 3 * nx // 4))
+        sz = int(rng.integers(4, max(5, nz // 4)))
+        src = S.default_source(x=sx, y=ny // 2, z=sz, kind=S.FORCE, amplitude=1.0e8, direction=(0.0, 0.0, 1.0))
+        nrec = min(receivers, nx)
+        rec = [(int(round(i * (nx - 1) / max(1, nrec - 1))), ny // 2, 0) for i in range(nrec)]
+        shots.append((ctx, (vp, vs, rho), src, rec))
+    return shots

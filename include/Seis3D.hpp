@@ -1,0 +1,248 @@
+// Seis3D.hpp - C++ solver-session API of the reference (cuda/Cuda_Seis3D.cuh:6-42 on the POD
+// types of include/Common3D.h:9-98), header-only, over the C ABI of seis3d.h.
+//
+// Callers of the reference keep their code: the same namespace, type and function names, the
+// same field names and defaults, the same argument meaning, and the same exception types with
+// the reference's message prefixes (std::invalid_argument for bad input, std::runtime_error for
+// device failures and non-finite wavefields, std::logic_error for lifecycle misuse).  What
+// differs is what the structs hold: GPUSimData3D owns an opaque session handle instead of raw
+// device pointers, since the device layout (pitched rows, x-fastest CPML slabs) is the
+// library's own.  Link with libseis3d.so.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "seis3d.h"
+
+namespace Seis3D {
+
+constexpr int stencilRadius3D = 4;
+constexpr std::size_t maxPhysicalNodes3D = std::size_t(300) * 300 * 400;
+constexpr int maxVolumeAxis3D = 256;
+constexpr int maxVolumeOutput3D = 1536;
+constexpr std::size_t memoryLimit3D = std::size_t(4) * 1024 * 1024 * 1024;
+constexpr int maxReceivers3D = 300;
+constexpr int maxSegySamples3D = 32767;
+constexpr int maxSegyIntervalUs3D = 32767;
+
+enum class Field3D : int { Vx, Vy, Vz, Sxx, Syy, Szz, Sxy, Syz, Sxz };
+enum class ModelKind3D : int { Uniform, TwoLayer, Sphere, Imported, ThreeLayer, DippingLayer,
+    LowVelocityLayer, Fault, Gradient, SaltDome };
+enum class SourceKind3D : int { Explosion, Force };
+
+// include/Common3D.h:24-42, same names and defaults.
+struct SimulationContext3D {
+    int nx3D = 300, ny3D = 50, nz3D = 300;
+    int pml3D = 12;
+    bool freeSurface3D = true;
+    bool reflecting3D = false;
+    float spacing3D = 10.0f, cfl3D = 0.85f, dt3D = 0.0f;
+    bool manualTimeStep3D = false;
+    float requestedDtMs3D = 0.5f;
+    float safeDt3D = 0.0f;
+    float vp3D = 3000.0f, vs3D = 1700.0f, density3D = 2200.0f;
+    float layerVp3D = 4200.0f, layerVs3D = 2400.0f, layerDensity3D = 2500.0f;
+    float maxVp3D = 4200.0f, frequency3D = 20.0f;
+    ModelKind3D modelKind3D = ModelKind3D::TwoLayer;
+    float interfaceDepth3D = 0.5f, layerDip3D = 0.25f, layerThickness3D = 0.15f;
+    float faultThrow3D = 0.18f, anomalyRadius3D = 0.187f;
+    int gx3D = 0, gy3D = 0, gz3D = 0, ox3D = 0, oy3D = 0, oz3D = 0;
+    std::size_t cells3D = 0;
+};
+
+struct Source3D {
+    int x3D = 48, y3D = 48, z3D = 24;
+    SourceKind3D kind3D = SourceKind3D::Explosion;
+    float amplitude3D = 1.0e8f;
+    float direction3D[3] = {0.0f, 0.0f, 1.0f};
+};
+
+struct Receiver3D { int x3D = 0, y3D = 0, z3D = 0; };
+
+struct MemoryBudget3D {
+    std::size_t fieldBytes3D = 0, cpmlBytes3D = 0, viewBytes3D = 0, totalBytes3D = 0;
+};
+
+// The caller owns the struct; the library owns the session behind it.
+struct GPUSimData3D {
+    seis3d_t* session3D = nullptr;
+    int receiverCount3D = 0;
+    std::size_t allocatedBytes3D = 0;
+};
+
+inline int activePml3D(const SimulationContext3D& c) { return c.reflecting3D ? 0 : c.pml3D; }
+
+inline const char* boundaryName3D(const SimulationContext3D& c) {
+    return c.reflecting3D ? "Reflecting slip walls" : c.freeSurface3D ? "Free top + CPML" : "All faces CPML";
+}
+
+inline std::size_t gridIndex3D(const SimulationContext3D& c, int x, int y, int z) {
+    return (std::size_t(z) * c.gy3D + y) * c.gx3D + x;
+}
+
+namespace detail {
+
+inline void check(int rc) {
+    if (rc == SEIS3D_OK) return;
+    const std::string message = seis3d_last_error();
+    if (rc == SEIS3D_EINVAL) throw std::invalid_argument(message);
+    if (rc == SEIS3D_ELIFECYCLE) throw std::logic_error(message);
+    throw std::runtime_error(message);
+}
+
+inline seis3d_ctx toC(const SimulationContext3D& c) {
+    seis3d_ctx o{};
+    o.nx = c.nx3D; o.ny = c.ny3D; o.nz = c.nz3D; o.pml = c.pml3D;
+    o.free_surface = c.freeSurface3D; o.reflecting = c.reflecting3D;
+    o.spacing = c.spacing3D; o.cfl = c.cfl3D; o.dt = c.dt3D;
+    o.manual_dt = c.manualTimeStep3D; o.requested_dt_ms = c.requestedDtMs3D; o.safe_dt = c.safeDt3D;
+    o.vp = c.vp3D; o.vs = c.vs3D; o.density = c.density3D;
+    o.layer_vp = c.layerVp3D; o.layer_vs = c.layerVs3D; o.layer_density = c.layerDensity3D;
+    o.max_vp = c.maxVp3D; o.frequency = c.frequency3D;
+    o.model_kind = int(c.modelKind3D);
+    o.interface_depth = c.interfaceDepth3D; o.layer_dip = c.layerDip3D; o.layer_thickness = c.layerThickness3D;
+    o.fault_throw = c.faultThrow3D; o.anomaly_radius = c.anomalyRadius3D;
+    o.gx = c.gx3D; o.gy = c.gy3D; o.gz = c.gz3D; o.ox = c.ox3D; o.oy = c.oy3D; o.oz = c.oz3D;
+    o.cells = c.cells3D;
+    return o;
+}
+
+inline void fromC(const seis3d_ctx& o, SimulationContext3D& c) {
+    c.nx3D = o.nx; c.ny3D = o.ny; c.nz3D = o.nz; c.pml3D = o.pml;
+    c.freeSurface3D = o.free_surface != 0; c.reflecting3D = o.reflecting != 0;
+    c.spacing3D = o.spacing; c.cfl3D = o.cfl; c.dt3D = o.dt;
+    c.manualTimeStep3D = o.manual_dt != 0; c.requestedDtMs3D = o.requested_dt_ms; c.safeDt3D = o.safe_dt;
+    c.vp3D = o.vp; c.vs3D = o.vs; c.density3D = o.density;
+    c.layerVp3D = o.layer_vp; c.layerVs3D = o.layer_vs; c.layerDensity3D = o.layer_density;
+    c.maxVp3D = o.max_vp; c.frequency3D = o.frequency;
+    c.modelKind3D = ModelKind3D(o.model_kind);
+    c.interfaceDepth3D = o.interface_depth; c.layerDip3D = o.layer_dip; c.layerThickness3D = o.layer_thickness;
+    c.faultThrow3D = o.fault_throw; c.anomalyRadius3D = o.anomaly_radius;
+    c.gx3D = o.gx; c.gy3D = o.gy; c.gz3D = o.gz; c.ox3D = o.ox; c.oy3D = o.oy; c.oz3D = o.oz;
+    c.cells3D = std::size_t(o.cells);
+}
+
+inline seis3d_source toC(const Source3D& s) {
+    seis3d_source o{};
+    o.x = s.x3D; o.y = s.y3D; o.z = s.z3D; o.kind = int(s.kind3D); o.amplitude = s.amplitude3D;
+    for (int i = 0; i < 3; ++i) o.direction[i] = s.direction3D[i];
+    return o;
+}
+
+inline seis3d_t* session(const GPUSimData3D& g) {
+    if (!g.session3D) throw std::logic_error("Seis3D: simulation not initialized.");
+    return g.session3D;
+}
+
+}  // namespace detail
+
+// validateContext3D (cuda/Cuda_Seis3D.cu:433-483).
+inline void validateContext3D(SimulationContext3D& c) {
+    seis3d_ctx o = detail::toC(c);
+    detail::check(seis3d_validate_ctx(&o));
+    detail::fromC(o, c);
+}
+
+// estimateMemory3D (cuda/Cuda_Seis3D.cu:504-516).
+inline MemoryBudget3D estimateMemory3D(const SimulationContext3D& c) {
+    const seis3d_ctx o = detail::toC(c);
+    seis3d_budget b{};
+    detail::check(seis3d_estimate_memory(&o, &b));
+    MemoryBudget3D m;
+    m.fieldBytes3D = b.field_bytes; m.cpmlBytes3D = b.cpml_bytes; m.viewBytes3D = b.view_bytes;
+    m.totalBytes3D = b.total_bytes;
+    return m;
+}
+
+// initializeGpu3D (cuda/Cuda_Seis3D.cu:519-555).
+inline void initializeGpu3D(GPUSimData3D& g, const SimulationContext3D& c) {
+    if (g.session3D) throw std::logic_error("Seis3D: release the previous session before initializing.");
+    const seis3d_ctx o = detail::toC(c);
+    seis3d_t* t = nullptr;
+    detail::check(seis3d_create(&o, &t));
+    g.session3D = t;
+    g.receiverCount3D = 0;
+    std::uint64_t bytes = 0;
+    if (seis3d_device_bytes(&o, &bytes) == SEIS3D_OK) g.allocatedBytes3D = std::size_t(bytes);
+}
+
+// releaseGpu3D (cuda/Cuda_Seis3D.cu:558-569): idempotent.
+inline void releaseGpu3D(GPUSimData3D& g) noexcept {
+    seis3d_destroy(g.session3D);
+    g = GPUSimData3D{};
+}
+
+// resetGpu3D (cuda/Cuda_Seis3D.cu:572-577).
+inline void resetGpu3D(GPUSimData3D& g, const SimulationContext3D&) { detail::check(seis3d_reset(detail::session(g))); }
+
+// refreshCpmlCoefficients3D (cuda/Cuda_Seis3D.cu:580-599).
+inline void refreshCpmlCoefficients3D(GPUSimData3D& g, const SimulationContext3D& c) {
+    const seis3d_ctx o = detail::toC(c);
+    detail::check(seis3d_refresh_cpml(detail::session(g), &o));
+}
+
+// runStep3D (cuda/Cuda_Seis3D.cu:602-624).
+inline void runStep3D(GPUSimData3D& g, const SimulationContext3D&, const Source3D& s, std::uint64_t step) {
+    const seis3d_source o = detail::toC(s);
+    detail::check(seis3d_step(detail::session(g), &o, step));
+}
+
+// Several runStep3D calls enqueued at once (the loop of advanceSimulation3D,
+// src/Seis3D_Sim.cpp:125-146), sampling into the device gather when recording.
+inline void runSteps3D(GPUSimData3D& g, const Source3D& s, std::uint64_t first, std::int64_t count) {
+    const seis3d_source o = detail::toC(s);
+    detail::check(seis3d_run(detail::session(g), &o, first, count));
+}
+
+// setReceivers3D (cuda/Cuda_Seis3D.cu:649-668).
+inline void setReceivers3D(GPUSimData3D& g, const SimulationContext3D&, const Receiver3D* r, int count) {
+    static_assert(sizeof(Receiver3D) == sizeof(seis3d_receiver), "receiver layout");
+    detail::check(seis3d_set_receivers(detail::session(g), reinterpret_cast<const seis3d_receiver*>(r), count));
+    g.receiverCount3D = count;
+}
+
+// sampleReceivers3D (cuda/Cuda_Seis3D.cu:671-681).
+inline const float* sampleReceivers3D(GPUSimData3D& g, const SimulationContext3D&) {
+    const float* out = nullptr;
+    detail::check(seis3d_sample_receivers(detail::session(g), &out));
+    return out;
+}
+
+// probeMaterial3D (cuda/Cuda_Seis3D.cu:684-693).
+inline void probeMaterial3D(GPUSimData3D& g, const SimulationContext3D&, const Receiver3D& p, float values[3]) {
+    const seis3d_receiver r{p.x3D, p.y3D, p.z3D};
+    detail::check(seis3d_probe(detail::session(g), &r, values));
+}
+
+// uploadModelPlane3D / finishModelUpload3D / downloadModelPlane3D (cuda/Cuda_Seis3D.cu:696-733).
+inline void uploadModelPlane3D(GPUSimData3D& g, const SimulationContext3D&, int plane, const float* values) {
+    detail::check(seis3d_upload_plane(detail::session(g), plane, values));
+}
+inline void finishModelUpload3D(GPUSimData3D& g, const SimulationContext3D&) {
+    detail::check(seis3d_finish_upload(detail::session(g)));
+}
+inline void downloadModelPlane3D(GPUSimData3D& g, const SimulationContext3D&, int plane, float* values) {
+    detail::check(seis3d_download_plane(detail::session(g), plane, values));
+}
+
+// Snapshot of one padded field (reference layout gx*gy*gz, x fastest); replaces reading
+// g.field3D[f] with cudaMemcpy in the reference's tests.
+inline std::vector<float> downloadField3D(GPUSimData3D& g, const SimulationContext3D& c, Field3D f) {
+    std::vector<float> out(std::size_t(c.gx3D) * c.gy3D * c.gz3D);
+    detail::check(seis3d_download_field(detail::session(g), int(f), out.data()));
+    return out;
+}
+
+inline void uploadField3D(GPUSimData3D& g, Field3D f, const std::vector<float>& values) {
+    detail::check(seis3d_upload_field(detail::session(g), int(f), values.data()));
+}
+
+// Wait for the session stream (the reference's cudaStreamSynchronize(g.stream3D)).
+inline void synchronize3D(GPUSimData3D& g) { detail::check(seis3d_synchronize(detail::session(g))); }
+
+}  // namespace Seis3D

@@ -229,10 +229,32 @@ int seis3d_run_timed(seis3d_t* t, const seis3d_source* src, uint64_t first_step,
 int seis3d_step_bytes(const seis3d_ctx* ctx, uint64_t* total, uint64_t* velocity,
                       uint64_t* stress, uint64_t* n_upd, uint64_t* n_psi);
 
+/* ---- Autotuner and multi-shot batches (no reference counterpart) ---------- */
+
+/* Time each candidate kernel shape for `steps` steps on a scratch session of
+ * the same grid, keep the fastest for this session and cache it per grid
+ * (padded dims, CPML, boundaries) for the process. */
+int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* family, int* tile_y,
+                    int* z_chunk, double* best_ms_per_step);
+
+typedef struct seis3d_batch seis3d_batch_t;
+/* Capture n_steps steps of every session (each with its source, receivers and
+ * an open gather) into one HIP graph: each shot runs on its own stream, the
+ * graph is instantiated once and replayed with one launch.  Replaces the
+ * sequential shot loop of the reference (src/Seis3D_Sim.cpp:133-143,
+ * src/Seis3D/Acquisition3D.cpp:233-242). */
+int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources, int count,
+                        int64_t n_steps, seis3d_batch_t** out);
+int seis3d_batch_launch(seis3d_batch_t* b);
+int seis3d_batch_synchronize(seis3d_batch_t* b);
+void seis3d_batch_destroy(seis3d_batch_t* b);
+
 /* ---- Errors and build info ------------------------------------------------ */
 
 const char* seis3d_last_error(void);
 const char* seis3d_version(void);
+/* Number of visible HIP devices (0 when none or the runtime is unusable). */
+int seis3d_device_count(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,33 @@
+"""Runs the C++ test programs built against include/Seis3D.hpp (tests/cpp).
+
+Like the reference's CTest programs they run their CPU checks first and exit 77 (skip) when no
+GPU is visible; on a GPU box the full program must pass (exit 0).
+"""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+BIN = Path(__file__).resolve().parent / "cpp" / "bin"
+
+
+def run(name, timeout):
+    exe = BIN / name
+    if not exe.exists():
+        pytest.fail(f"{exe} not built (run __graft_entry__.build())")
+    return subprocess.run([str(exe)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_boundary_regression_cpu_checks():
+    """Validation and budget checks run everywhere; the GPU part may skip with 77."""
+    r = run("seis3d_boundary_regression", 600)
+    assert r.returncode in (0, 77), r.stderr + r.stdout[-2000:]
+    assert "BUDGET 96/96/96" in r.stdout
+
+
+@pytest.mark.gpu
+def test_boundary_regression_gpu(gpu):
+    r = run("seis3d_boundary_regression", 600)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert "PASS" in r.stdout
