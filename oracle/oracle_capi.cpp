@@ -175,6 +175,22 @@ int oracle3d_sample_scalar(Handle* h, int x, int y, int z, int component, double
     return guard([&] { with_solver(h, [&](auto& s) { *out = double(s.sample_scalar(x, y, z, component)); }); });
 }
 
+// Misfit and gradient (vp, vs, rho on the physical grid [z][y][x]) of
+// n_steps steps sampled every `stride`; observed is [sample][receiver][3].
+int oracle3d_fwi_gradient(Handle* h, const seis3d_source* src, int64_t n_steps, int stride, const double* observed,
+                          double* misfit, double* g_vp, double* g_vs, double* g_rho) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            using Real = std::decay_t<decltype(s.f[0][0])>;
+            const std::size_t nobs = std::size_t(n_steps / stride) * s.receivers.size() * 3;
+            const std::size_t nphys = std::size_t(s.c.nx) * s.c.ny * s.c.nz;
+            std::vector<Real> obs(observed, observed + nobs), a(nphys), b(nphys), c(nphys);
+            *misfit = double(s.fwi_gradient(*src, n_steps, stride, obs.data(), a.data(), b.data(), c.data()));
+            for (std::size_t k = 0; k < nphys; ++k) { g_vp[k] = double(a[k]); g_vs[k] = double(b[k]); g_rho[k] = double(c[k]); }
+        });
+    });
+}
+
 int oracle3d_nonfinite(Handle* h) {
     return with_solver(h, [](auto& s) { return s.nonfinite ? 1 : 0; });
 }

@@ -59,6 +59,8 @@ def library() -> C.CDLL:
             "oracle3d_set_field": (C.c_int, [vp, C.c_int, P(C.c_double)]),
             "oracle3d_sample_scalar": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
             "oracle3d_nonfinite": (C.c_int, [vp]),
+            "oracle3d_fwi_gradient": (C.c_int, [vp, P(Source), C.c_int64, C.c_int, P(C.c_double), P(C.c_double),
+                                                P(C.c_double), P(C.c_double), P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -207,6 +209,17 @@ class Oracle3D:
         out = C.c_double()
         _check(library().oracle3d_sample_scalar(self._h, x, y, z, component, C.byref(out)))
         return out.value
+
+    def fwi_gradient(self, src: Source, n_steps: int, stride: int, observed: np.ndarray):
+        """Misfit and its gradient w.r.t. vp, vs, rho ([nz, ny, nx]) by the discrete adjoint."""
+        c = self.ctx
+        obs = np.ascontiguousarray(observed, dtype=np.float64).reshape(-1)
+        shape = (c.nz, c.ny, c.nx)
+        g = [np.zeros(shape) for _ in range(3)]
+        m = C.c_double()
+        _check(library().oracle3d_fwi_gradient(self._h, C.byref(src), int(n_steps), int(stride), _dptr(obs),
+                                               C.byref(m), *[_dptr(a) for a in g]))
+        return m.value, g[0], g[1], g[2]
 
     @property
     def nonfinite(self) -> bool:

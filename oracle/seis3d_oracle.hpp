@@ -503,6 +503,246 @@ public:
             out[3 * r + 2] = sample_scalar(p.x, p.y, p.z, 0);
         }
     }
+
+    // ------------------------------------------------------------------
+    // Discrete adjoint of one step (no reference counterpart).
+    //
+    // The step is linear in (v, sigma, psi) for a fixed model, so its adjoint
+    // is the transpose of the same operations in reverse order: the stencil
+    // transposes are D+^T = -D- and D-^T = -D+, the CPML recurrence
+    // psi' = b psi + a d, e = d/kappa + psi' transposes to
+    // L_tot = L_psi' + L_e, L_d = L_e/kappa + a L_tot, L_psi = b L_tot,
+    // and the constitutive law to L_e = dt M^T L_sigma.  Model gradients are
+    // taken with respect to 1/rho, lambda and mu at every padded cell,
+    // including the harmonic face buoyancy 2ab/(a+b) and the 4-cell harmonic
+    // shear modulus.  All-CPML boundaries only (no free surface, no walls).
+    // ------------------------------------------------------------------
+    struct Snapshot {
+        std::vector<Real> f[9];
+        std::vector<Real> cpml[3];
+    };
+    Snapshot save() const {
+        Snapshot s;
+        for (int k = 0; k < 9; ++k) s.f[k] = f[k];
+        for (int a = 0; a < 3; ++a) s.cpml[a] = cpml[a];
+        return s;
+    }
+    void load(const Snapshot& s) {
+        for (int k = 0; k < 9; ++k) f[k] = s.f[k];
+        for (int a = 0; a < 3; ++a) cpml[a] = s.cpml[a];
+    }
+    struct Adjoint {
+        std::vector<Real> f[9];      // adjoints of the nine wavefields
+        std::vector<Real> cpml[3];   // adjoints of the CPML memories (same slab layout)
+        std::vector<Real> g_ib, g_lam, g_mu;
+    };
+    Adjoint make_adjoint() const {
+        Adjoint A;
+        for (auto& v : A.f) v.assign(c.cells, Real(0));
+        for (int a = 0; a < 3; ++a) A.cpml[a].assign(cpml[a].size(), Real(0));
+        A.g_ib.assign(c.cells, Real(0)); A.g_lam.assign(c.cells, Real(0)); A.g_mu.assign(c.cells, Real(0));
+        return A;
+    }
+
+    // Slab position of a derivative, -1 when outside the axis' CPML slab.
+    long long slab_slot_index(int x, int y, int z, int axis, int slot, int& coordinate) const {
+        const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
+        coordinate = axis == 0 ? x : axis == 1 ? y : z;
+        const bool topFree = axis == 2 && c.free_surface;
+        int q = -1;
+        if (!topFree && coordinate < kRadius + c.pml) q = coordinate - kRadius;
+        else if (coordinate >= n - kRadius - c.pml) q = (topFree ? 0 : c.pml) + coordinate - (n - kRadius - c.pml);
+        if (q < 0 || c.reflecting) return -1;
+        const int thickness = (topFree ? 1 : 2) * c.pml;
+        const std::size_t slab = axis == 0 ? (std::size_t(z) * c.gy + y) * thickness + q
+            : axis == 1 ? (std::size_t(z) * c.gx + x) * thickness + q
+            : (std::size_t(y) * c.gx + x) * thickness + q;
+        return (long long)(std::size_t(slot) * slab_cells[axis] + slab);
+    }
+
+    // Filtered derivative e = d/kappa + psi' of a stored state (psi' as stored).
+    Real filtered(const Snapshot& after, Real d, int x, int y, int z, int axis, int slot, bool half) const {
+        int coordinate = 0;
+        const long long k = slab_slot_index(x, y, z, axis, slot, coordinate);
+        if (k < 0) return d;
+        const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
+        const Real* kk = coef[axis].data() + (std::size_t(half) * n + coordinate) * 3;
+        return kk[2] * d + after.cpml[axis][std::size_t(k)];
+    }
+
+    // Adjoint of the CPML filter for one derivative: returns L_d and updates L_psi.
+    Real adjoint_filter(Adjoint& A, Real le, int x, int y, int z, int axis, int slot, bool half) const {
+        int coordinate = 0;
+        const long long k = slab_slot_index(x, y, z, axis, slot, coordinate);
+        if (k < 0) return le;
+        const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
+        const Real* kk = coef[axis].data() + (std::size_t(half) * n + coordinate) * 3;
+        Real& lp = A.cpml[axis][std::size_t(k)];
+        const Real total = lp + le;
+        lp = kk[1] * total;
+        return kk[2] * le + kk[0] * total;
+    }
+
+    // Adjoint of one full step n (state `before` -> `after`), with the misfit
+    // residual of a sample taken after this step already added to A.f[0..2].
+    void adjoint_step(const Snapshot& before, const Snapshot& after, Adjoint& A, const seis3d_source& s,
+                      std::uint64_t n) {
+        const std::size_t st[3] = {1, std::size_t(c.gx), std::size_t(c.gx) * c.gy};
+        const Real invH = Real(1) / Real(c.spacing), dt = Real(c.dt);
+        std::vector<Real> ld[9];
+        for (auto& v : ld) v.assign(c.cells, Real(0));
+        // ---- stress half-step ----
+        for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, z);
+            Real e[3][3];
+            for (int v = 0; v < 3; ++v) for (int a = 0; a < 3; ++a) {
+                const Real d = a == v ? derivative<false>(after.f[v].data(), i, st[a], invH)
+                                      : derivative<true>(after.f[v].data(), i, st[a], invH);
+                e[v][a] = filtered(after, d, x, y, z, a, 3 + v, a != v);
+            }
+            const Real lam = lambda[i], m = mu[i];
+            const Real Ls[6] = {A.f[3][i], A.f[4][i], A.f[5][i], A.f[6][i], A.f[7][i], A.f[8][i]};
+            const Real sumL = Ls[0] + Ls[1] + Ls[2];
+            Real le[3][3] = {};
+            for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+            const Real mxy = shear_mu(i, st[0], st[1]), myz = shear_mu(i, st[1], st[2]), mxz = shear_mu(i, st[0], st[2]);
+            le[0][1] = le[1][0] = dt * mxy * Ls[3];
+            le[1][2] = le[2][1] = dt * myz * Ls[4];
+            le[0][2] = le[2][0] = dt * mxz * Ls[5];
+            const Real tr = e[0][0] + e[1][1] + e[2][2];
+            A.g_lam[i] += dt * tr * sumL;
+            A.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+            const Real gh[3] = {dt * (e[0][1] + e[1][0]) * Ls[3], dt * (e[1][2] + e[2][1]) * Ls[4],
+                                dt * (e[0][2] + e[2][0]) * Ls[5]};
+            const std::size_t pairs[3][2] = {{st[0], st[1]}, {st[1], st[2]}, {st[0], st[2]}};
+            const Real mh[3] = {mxy, myz, mxz};
+            for (int p = 0; p < 3; ++p) {
+                if (mh[p] == 0) continue;
+                const std::size_t cells4[4] = {i, i + pairs[p][0], i + pairs[p][1], i + pairs[p][0] + pairs[p][1]};
+                for (auto k : cells4) A.g_mu[k] += gh[p] * mh[p] * mh[p] / (4 * mu[k] * mu[k]);
+            }
+            for (int v = 0; v < 3; ++v) for (int a = 0; a < 3; ++a)
+                ld[3 * v + a][i] = adjoint_filter(A, le[v][a], x, y, z, a, 3 + v, a != v);
+        }
+        // L_v += D^T L_d (transpose of D- for a == v, of D+ otherwise).
+        for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, z);
+            for (int v = 0; v < 3; ++v) for (int a = 0; a < 3; ++a) {
+                const Real t = a == v ? derivative<true>(ld[3 * v + a].data(), i, st[a], invH)
+                                      : derivative<false>(ld[3 * v + a].data(), i, st[a], invH);
+                A.f[v][i] -= t;
+            }
+        }
+        for (auto& v : ld) std::fill(v.begin(), v.end(), Real(0));
+        // ---- velocity half-step ----
+        const int stressFields[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
+        const Real rick = ricker(c.frequency, double(n) * c.dt);
+        const Real norm = Real(10.8387785335f);
+        for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, z);
+            for (int v = 0; v < 3; ++v) {
+                Real E = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const Real* fld = before.f[stressFields[v][a]].data();
+                    const Real d = a == v ? derivative<true>(fld, i, st[a], invH) : derivative<false>(fld, i, st[a], invH);
+                    E += filtered(after, d, x, y, z, a, v, a == v);
+                }
+                const Real lv = A.f[v][i];
+                const Real aa = inv_rho[i], bb = inv_rho[i + st[v]];
+                const Real h = 2 * aa * bb / (aa + bb);
+                Real dvdh = dt * E;
+                if (s.kind == SEIS3D_FORCE) {
+                    const int dx = x - (s.x + c.ox), dy = y - (s.y + c.oy), dz = z - (s.z + c.oz);
+                    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+                        const Real w = Real(std::exp(Real(-0.5) * Real(dx * dx + dy * dy + dz * dz)));
+                        dvdh += Real(c.dt) * Real(s.amplitude) * rick * w / norm * Real(s.direction[v]);
+                    }
+                }
+                const Real gh = dvdh * lv;
+                const Real den = (aa + bb) * (aa + bb);
+                A.g_ib[i] += gh * 2 * bb * bb / den;
+                A.g_ib[i + st[v]] += gh * 2 * aa * aa / den;
+                const Real le = dt * h * lv;
+                for (int a = 0; a < 3; ++a) ld[3 * v + a][i] = adjoint_filter(A, le, x, y, z, a, v, a == v);
+            }
+        }
+        // L_sigma += D^T L_d (transpose of D+ for a == v, of D- otherwise).
+        for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, z);
+            for (int v = 0; v < 3; ++v) for (int a = 0; a < 3; ++a) {
+                const Real t = a == v ? derivative<false>(ld[3 * v + a].data(), i, st[a], invH)
+                                      : derivative<true>(ld[3 * v + a].data(), i, st[a], invH);
+                A.f[stressFields[v][a]][i] -= t;
+            }
+        }
+    }
+
+    // Residual injection R^T r for a sample (collocation adjoint).
+    void inject_residual(Adjoint& A, const Real* r) const {
+        const std::size_t st[3] = {1, std::size_t(c.gx), std::size_t(c.gx) * c.gy};
+        const int comp[3] = {0, 1, 2};   // r is [vx, vy, vz]
+        for (std::size_t k = 0; k < receivers.size(); ++k) {
+            const auto p = receivers[k];
+            const auto i = index(p.x + c.ox, p.y + c.oy, p.z + c.oz);
+            for (int v = 0; v < 3; ++v) {
+                const Real h = Real(0.5) * r[3 * k + comp[v]];
+                A.f[v][i] += h;
+                A.f[v][i - st[v]] += h;
+            }
+        }
+    }
+
+    // Misfit 1/2 sum (gather - observed)^2 and its gradient with respect to
+    // vp, vs, rho at every physical cell ([z][y][x]), storing every state.
+    Real fwi_gradient(const seis3d_source& s, std::int64_t steps, int stride, const Real* observed,
+                      Real* g_vp, Real* g_vs, Real* g_rho) {
+        if (c.free_surface || c.reflecting) throw Invalid("Seis3D: the adjoint supports all-CPML boundaries only.");
+        reset();
+        std::vector<Snapshot> states;
+        states.reserve(std::size_t(steps) + 1);
+        states.push_back(save());
+        const std::size_t nr = receivers.size();
+        std::vector<Real> gather(std::size_t(steps / stride) * nr * 3);
+        int count = 0;
+        for (std::int64_t n = 0; n < steps; ++n) {
+            step(s, std::uint64_t(n));
+            states.push_back(save());
+            if ((n + 1) % stride == 0) sample(gather.data() + std::size_t(count++) * nr * 3);
+        }
+        Real misfit = 0;
+        std::vector<Real> res(gather.size());
+        for (std::size_t k = 0; k < gather.size(); ++k) {
+            res[k] = gather[k] - observed[k];
+            misfit += Real(0.5) * res[k] * res[k];
+        }
+        Adjoint A = make_adjoint();
+        for (std::int64_t n = steps - 1; n >= 0; --n) {
+            if ((n + 1) % stride == 0) inject_residual(A, res.data() + std::size_t((n + 1) / stride - 1) * nr * 3);
+            adjoint_step(states[std::size_t(n)], states[std::size_t(n) + 1], A, s, std::uint64_t(n));
+        }
+        load(states.back());
+        // Fold the padding (materials are clamp-extended copies) onto physical cells.
+        std::vector<Real> gib(std::size_t(c.nx) * c.ny * c.nz, 0), glam(gib.size(), 0), gmu(gib.size(), 0);
+        for (std::size_t i = 0; i < c.cells; ++i) {
+            const int x = int(i % c.gx), y = int((i / c.gx) % c.gy), z = int(i / (std::size_t(c.gx) * c.gy));
+            const int px = std::max(0, std::min(c.nx - 1, x - c.ox));
+            const int py = std::max(0, std::min(c.ny - 1, y - c.oy));
+            const int pz = std::max(0, std::min(c.nz - 1, z - c.oz));
+            const std::size_t j = (std::size_t(pz) * c.ny + py) * c.nx + px;
+            gib[j] += A.g_ib[i]; glam[j] += A.g_lam[i]; gmu[j] += A.g_mu[i];
+        }
+        for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
+            const std::size_t j = (std::size_t(z) * c.ny + y) * c.nx + x;
+            const auto i = index(x + c.ox, y + c.oy, z + c.oz);
+            const Real rho = Real(1) / inv_rho[i];
+            const Real vs2 = mu[i] / rho, vp2 = (lambda[i] + 2 * mu[i]) / rho;
+            const Real vp = std::sqrt(vp2), vs = std::sqrt(vs2);
+            g_vp[j] = glam[j] * 2 * rho * vp;
+            g_vs[j] = glam[j] * (-4 * rho * vs) + gmu[j] * 2 * rho * vs;
+            g_rho[j] = glam[j] * (vp2 - 2 * vs2) + gmu[j] * vs2 - gib[j] / (rho * rho);
+        }
+        return misfit;
+    }
 };
 
 // fitSurvey3D (src/Seis3D_Sim.cpp:42-60).
