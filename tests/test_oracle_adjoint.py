@@ -110,7 +110,7 @@ def test_mismatch_is_second_order(request, name, param, seed):
 
 def test_gradient_rejects_free_surface():
     vp, vs, rho = W.gradient_random_model(16, 16, 16, seed=3, corr=3.0, amplitude=0.05)
-    ctx = W.imported_context(16, 16, 16, max_vp=float(vp.max()), pml=4, free_surface=True, frequency=40.0)
+    ctx = W.imported_context(16, 16, 16, max_vp=float(vp.max()), pml=8, free_surface=True, frequency=40.0)
     with O.Oracle3D(ctx, 64) as o:
         o.upload_model(vp, vs, rho)
         o.set_receivers([(5, 5, 0)])
