@@ -32,7 +32,8 @@ HIP_FLAGS = [
 ]
 
 LIBRARIES = {
-    "libseis3d.so": ["seis3d_kernels.hip", "seis3d_pipe.hip", "seis3d_column.hip", "seis3d_dma.hip", "seis3d_api.cpp"],
+    "libseis3d.so": ["seis3d_kernels.hip", "seis3d_pipe.hip", "seis3d_column.hip", "seis3d_dma.hip", "seis3d_adjoint.hip",
+                    "seis3d_api.cpp", "seis3d_fwi.cpp"],
 }
 
 

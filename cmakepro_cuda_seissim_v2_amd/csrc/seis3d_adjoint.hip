@@ -1,0 +1,423 @@
+// seis3d_adjoint.hip - discrete adjoint of the 3D O(8) velocity-stress step
+// (no reference counterpart): the FWI gradient with respect to 1/rho, lambda
+// and mu, folded and chain-ruled to vp, vs and rho.
+//
+// For a fixed model one step is linear in (v, sigma, psi), so its adjoint is
+// the transpose of the same operations in reverse order:
+//  * stencils: D+^T = -D-, D-^T = -D+ (the O(8) weights are antisymmetric);
+//  * CPML: psi' = b psi + a d, e = d/kappa + psi' transposes to
+//    L_tot = L_psi' + L_e, L_d = L_e/kappa + a L_tot, L_psi = b L_tot;
+//  * constitutive law: L_e = dt M^T L_sigma; momentum: L_e = dt h L_v.
+// Each half-step is a local pass (filtered adjoint derivatives L_d into nine
+// temporaries, CPML adjoint memories, gradient sums) followed by a gather
+// pass applying the transposed stencils.  Gradients are summed over time per
+// cell in their cheapest form (dJ/dlambda, dJ/dmu of the normal stresses,
+// dJ/dmu_h of each harmonic shear modulus, dJ/dh of each face buoyancy); the
+// harmonic-mean derivatives are applied once at the end.
+// Built with -ffp-contract=off, as the forward kernels.
+#include "seis3d_adjoint.hpp"
+#include "seis3d_device.hpp"
+
+namespace seis3d {
+
+namespace {
+
+__device__ __forceinline__ long long acell(const DevGrid& g, int x, int y, int z) {
+    return (static_cast<long long>(z) * g.gy + y) * g.px + x;
+}
+
+template <bool plus>
+__device__ __forceinline__ float aderiv(const float* __restrict__ f, long long i, long long s, float invh) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = f[i + (j - (plus ? 3 : 4)) * s];
+    return stencil8(v, invh);
+}
+
+__device__ __forceinline__ int source_r2a(const DevSource& s, int x, int y, int z) {
+    const int dx = x - s.x, dy = y - s.y, dz = z - s.z;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return -1;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// CPML slot position of a derivative along `axis` at (x, y, z), -1 outside the slab.
+__device__ __forceinline__ long long slab_index(const DevGrid& g, int axis, int slot, int q, int x, int y, int z) {
+    if (q < 0) return -1;
+    if (axis == 0) return psi_index_x(g, slot, q, y, z);
+    if (axis == 1) return psi_index_y(g, slot, q, x, z);
+    return psi_index_z(g, slot, q, x, y);
+}
+
+// Filtered derivative of a stored state: e = d/kappa + psi' (psi' after the step).
+__device__ __forceinline__ float filtered(const float* __restrict__ psi, long long idx, const float* __restrict__ k,
+                                          float d) {
+    return k[2] * d + psi[idx];
+}
+
+// Adjoint of the CPML filter of one derivative: returns L_d, updates L_psi.
+__device__ __forceinline__ float adjoint_filter(float* __restrict__ apsi, long long idx, const float* __restrict__ k,
+                                                float le) {
+    const float total = apsi[idx] + le;
+    apsi[idx] = k[1] * total;
+    return k[2] * le + k[0] * total;
+}
+
+struct Cell {
+    int x, y, z;
+    long long i;
+    int q[3];
+};
+
+__device__ __forceinline__ bool updated_cell(const DevGrid& g, Cell& c) {
+    c.x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    c.y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    c.z = int(blockIdx.z) + 4;
+    if (c.x >= g.gx - 4 || c.y >= g.gy - 4 || c.z >= g.gz - 4) return false;
+    c.i = acell(g, c.x, c.y, c.z);
+    c.q[0] = slab_q(c.x, g.gx, g.pml, false);
+    c.q[1] = slab_q(c.y, g.gy, g.pml, false);
+    c.q[2] = slab_q(c.z, g.gz, g.pml, g.free_surface != 0);
+    return true;
+}
+
+__device__ __forceinline__ const float* coef_row(const AdjStep& P, int axis, int half, const Cell& c) {
+    const int n = axis == 0 ? P.g.gx : axis == 1 ? P.g.gy : P.g.gz;
+    const int coord = axis == 0 ? c.x : axis == 1 ? c.y : c.z;
+    return P.coef[axis] + (half * n + coord) * 3;
+}
+
+// ---------------------------------------------------------------------------
+// Stress half-step adjoint
+// ---------------------------------------------------------------------------
+
+// Local pass: strain rates of the stored velocities, gradient sums, and the
+// filtered adjoint derivatives L_d[v][a] = adjoint CPML of dt M^T L_sigma.
+__global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
+    Cell c;
+    if (!updated_cell(P.g, c)) return;
+    const long long st[3] = {1, P.g.px, P.g.plane};
+    float e[3][3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float d = a == v ? aderiv<false>(P.fv[v], c.i, st[a], P.invh) : aderiv<true>(P.fv[v], c.i, st[a], P.invh);
+            const long long k = slab_index(P.g, a, 3 + v, c.q[a], c.x, c.y, c.z);
+            if (k >= 0) d = filtered(P.psi[a], k, coef_row(P, a, a != v ? 1 : 0, c), d);
+            e[v][a] = d;
+        }
+    const float lam = P.lambda[c.i], m = P.mu[c.i];
+    float Ls[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Ls[k] = P.a[3 + k][c.i];
+    const float sumL = Ls[0] + Ls[1] + Ls[2];
+    const float dt = P.dt;
+    const float* r = P.inv_mu;
+    const long long i = c.i;
+    const float mxy = shear_from_inverse(r[i], r[i + st[0]], r[i + st[1]], r[i + st[0] + st[1]]);
+    const float myz = shear_from_inverse(r[i], r[i + st[1]], r[i + st[2]], r[i + st[1] + st[2]]);
+    const float mxz = shear_from_inverse(r[i], r[i + st[0]], r[i + st[2]], r[i + st[0] + st[2]]);
+    float le[3][3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+    le[0][1] = le[1][0] = dt * mxy * Ls[3];
+    le[1][2] = le[2][1] = dt * myz * Ls[4];
+    le[0][2] = le[2][0] = dt * mxz * Ls[5];
+    const float tr = e[0][0] + e[1][1] + e[2][2];
+    P.g_lam[i] += dt * tr * sumL;
+    P.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+    P.g_shear[0][i] += dt * (e[0][1] + e[1][0]) * Ls[3];
+    P.g_shear[1][i] += dt * (e[1][2] + e[2][1]) * Ls[4];
+    P.g_shear[2][i] += dt * (e[0][2] + e[2][0]) * Ls[5];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float ld = le[v][a];
+            const long long k = slab_index(P.g, a, 3 + v, c.q[a], c.x, c.y, c.z);
+            if (k >= 0) ld = adjoint_filter(P.apsi[a], k, coef_row(P, a, a != v ? 1 : 0, c), ld);
+            P.t[3 * v + a][i] = ld;
+        }
+}
+
+// Gather pass: L_v -= D^T-stencils of L_d (D- for a == v was used forward, so D+ here).
+__global__ __launch_bounds__(256) void adjoint_stress_gather(AdjStep P) {
+    Cell c;
+    if (!updated_cell(P.g, c)) return;
+    const long long st[3] = {1, P.g.px, P.g.plane};
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        float acc = P.a[v][c.i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float t = a == v ? aderiv<true>(P.t[3 * v + a], c.i, st[a], P.invh)
+                                   : aderiv<false>(P.t[3 * v + a], c.i, st[a], P.invh);
+            acc -= t;
+        }
+        P.a[v][c.i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity half-step adjoint
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void adjoint_velocity_local(AdjStep P) {
+    Cell c;
+    if (!updated_cell(P.g, c)) return;
+    const long long st[3] = {1, P.g.px, P.g.plane};
+    const int sf[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};   // stress index - 3
+    const int r2 = P.src.active ? source_r2a(P.src, c.x, c.y, c.z) : -1;
+    const float dt = P.dt;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        float E = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float* fld = P.fs[sf[v][a]];
+            float d = a == v ? aderiv<true>(fld, c.i, st[a], P.invh) : aderiv<false>(fld, c.i, st[a], P.invh);
+            const long long k = slab_index(P.g, a, v, c.q[a], c.x, c.y, c.z);
+            if (k >= 0) d = filtered(P.psi[a], k, coef_row(P, a, a == v ? 1 : 0, c), d);
+            E += d;
+        }
+        const float lv = P.a[v][c.i];
+        const float h = face_buoyancy(P.inv_rho[c.i], P.inv_rho[c.i + st[v]]);
+        float dvdh = dt * E;
+        if (r2 >= 0) dvdh += (P.src.scale * P.src.w[r2] / P.src.norm) * P.src.dir[v];
+        P.g_buoy[v][c.i] += dvdh * lv;
+        const float le = dt * h * lv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float ld = le;
+            const long long k = slab_index(P.g, a, v, c.q[a], c.x, c.y, c.z);
+            if (k >= 0) ld = adjoint_filter(P.apsi[a], k, coef_row(P, a, a == v ? 1 : 0, c), ld);
+            P.t[3 * v + a][c.i] = ld;
+        }
+    }
+}
+
+// Gather pass: L_sigma -= transposed stencils of L_d, in the (v, a) order of the forward sum.
+__global__ __launch_bounds__(256) void adjoint_velocity_gather(AdjStep P) {
+    Cell c;
+    if (!updated_cell(P.g, c)) return;
+    const long long st[3] = {1, P.g.px, P.g.plane};
+    const int sf[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+    float s[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = P.a[3 + k][c.i];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float t = a == v ? aderiv<false>(P.t[3 * v + a], c.i, st[a], P.invh)
+                                   : aderiv<true>(P.t[3 * v + a], c.i, st[a], P.invh);
+            s[sf[v][a]] -= t;
+        }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P.a[3 + k][c.i] = s[k];
+}
+
+// R^T r: the receiver collocation 0.5 (v[i] + v[i - e_v]) transposed.
+__global__ void adjoint_inject(AdjStep P, const int* __restrict__ rec, int count, const float* __restrict__ res) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= 3 * count) return;
+    const int r = k / 3, v = k % 3;
+    const long long i = acell(P.g, rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+    const long long st = v == 0 ? 1 : v == 1 ? P.g.px : P.g.plane;
+    const float h = 0.5f * res[3 * r + v];
+    atomicAdd(P.a[v] + i, h);
+    atomicAdd(P.a[v] + i - st, h);
+}
+
+__global__ void residual_kernel(const float* __restrict__ g, const float* __restrict__ o, float* __restrict__ r,
+                                long long n) {
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) r[k] = g[k] - o[k];
+}
+
+// ---------------------------------------------------------------------------
+// Finalisation
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool interior(const DevGrid& g, int x, int y, int z) {
+    return x >= 4 && x < g.gx - 4 && y >= 4 && y < g.gy - 4 && z >= 4 && z < g.gz - 4;
+}
+
+// Per padded cell k: dJ/d(1/rho)[k] from the two faces that use rho[k] on each
+// axis (d(2ab/(a+b))/da = 2 b^2/(a+b)^2), and dJ/dmu[k] including the four
+// harmonic shear moduli of each plane that use mu[k]
+// (d(4/sum 1/mu_j)/dmu_k = mu_h^2 / (4 mu_k^2)).
+__global__ void adjoint_cells(AdjStep P, float* __restrict__ out_ib, float* __restrict__ out_mu) {
+    const DevGrid& g = P.g;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = int(blockIdx.y);
+    const int z = int(blockIdx.z);
+    if (x >= g.gx) return;
+    const long long k = acell(g, x, y, z);
+    const long long st[3] = {1, g.px, g.plane};
+    const int p[3] = {x, y, z};
+    float gib = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        if (interior(g, x, y, z)) {
+            const float aa = P.inv_rho[k], bb = P.inv_rho[k + st[v]];
+            const float den = (aa + bb) * (aa + bb);
+            gib += P.g_buoy[v][k] * 2 * bb * bb / den;
+        }
+        int q[3] = {p[0], p[1], p[2]};
+        q[v] -= 1;
+        if (interior(g, q[0], q[1], q[2])) {
+            const long long km = k - st[v];
+            const float aa = P.inv_rho[km], bb = P.inv_rho[k];
+            const float den = (aa + bb) * (aa + bb);
+            gib += P.g_buoy[v][km] * 2 * aa * aa / den;
+        }
+    }
+    float gmu = interior(g, x, y, z) ? P.g_mu[k] : 0.0f;
+    const int pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    const float* r = P.inv_mu;
+    const float mk = P.mu[k];
+#pragma unroll
+    for (int pp = 0; pp < 3; ++pp) {
+        const int A = pairs[pp][0], B = pairs[pp][1];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[A] -= o & 1;
+            q[B] -= o >> 1;
+            if (!interior(g, q[0], q[1], q[2])) continue;
+            const long long i = acell(g, q[0], q[1], q[2]);
+            const float G = P.g_shear[pp][i];
+            const float mh = shear_from_inverse(r[i], r[i + st[A]], r[i + st[B]], r[i + st[A] + st[B]]);
+            if (mh != 0.0f) gmu += G * mh * mh / (4 * mk * mk);
+        }
+    }
+    out_ib[k] = gib;
+    out_mu[k] = gmu;
+}
+
+// Folds the clamp-extended padding onto each physical cell (in padded raster
+// order) and applies the chain rule lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2,
+// 1/rho.
+__global__ void adjoint_fold(FoldParams F) {
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = int(blockIdx.y);
+    const int z = int(blockIdx.z);
+    if (x >= F.nx) return;
+    const DevGrid& g = F.g;
+    const int x0 = x == 0 ? 0 : x + F.ox, x1 = x == F.nx - 1 ? g.gx - 1 : x + F.ox;
+    const int y0 = y == 0 ? 0 : y + F.oy, y1 = y == F.ny - 1 ? g.gy - 1 : y + F.oy;
+    const int z0 = z == 0 ? 0 : z + F.oz, z1 = z == F.nz - 1 ? g.gz - 1 : z + F.oz;
+    float gib = 0.0f, glam = 0.0f, gmu = 0.0f;
+    for (int zz = z0; zz <= z1; ++zz)
+        for (int yy = y0; yy <= y1; ++yy)
+            for (int xx = x0; xx <= x1; ++xx) {
+                const long long i = acell(g, xx, yy, zz);
+                gib += F.g_ib[i];
+                glam += F.g_lam[i];
+                gmu += F.g_mu[i];
+            }
+    const long long i = acell(g, x + F.ox, y + F.oy, z + F.oz);
+    const float rho = 1.0f / F.inv_rho[i];
+    const float vs2 = F.mu[i] / rho, vp2 = (F.lambda[i] + 2 * F.mu[i]) / rho;
+    const float vp = sqrtf(vp2), vs = sqrtf(vs2);
+    const long long j = (static_cast<long long>(z) * F.ny + y) * F.nx + x;
+    F.g_vp[j] = glam * 2 * rho * vp;
+    F.g_vs[j] = glam * (-4 * rho * vs) + gmu * 2 * rho * vs;
+    F.g_rho[j] = glam * (vp2 - 2 * vs2) + gmu * vs2 - gib / (rho * rho);
+}
+
+// ---------------------------------------------------------------------------
+// Array utilities (fills instead of hipMemsetAsync, so they are graph-safe)
+// ---------------------------------------------------------------------------
+
+template <int OP>
+__global__ void array_kernel(ArrayBatch b) {
+    const int k = int(blockIdx.y);
+    const long long n = b.n[k];
+    float* __restrict__ d = b.dst[k];
+    const float* __restrict__ s = b.src[k];
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    const long long n4 = n / 4;
+    float4* d4 = reinterpret_cast<float4*>(d);
+    const float4* s4 = reinterpret_cast<const float4*>(s);
+    for (long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; j < n4; j += stride) {
+        if (OP == 0) d4[j] = s4[j];
+        if (OP == 1) d4[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (OP == 2) {
+            float4 a = d4[j];
+            const float4 c = s4[j];
+            a.x += c.x; a.y += c.y; a.z += c.z; a.w += c.w;
+            d4[j] = a;
+        }
+    }
+    for (long long j = 4 * n4 + static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += stride) {
+        if (OP == 0) d[j] = s[j];
+        if (OP == 1) d[j] = 0.0f;
+        if (OP == 2) d[j] += s[j];
+    }
+}
+
+template <int OP>
+hipError_t launch_arrays(const ArrayBatch& b, hipStream_t s) {
+    if (b.count <= 0) return hipSuccess;
+    if (b.count > 16) return hipErrorInvalidValue;
+    for (int k = 0; k < b.count; ++k)
+        if ((reinterpret_cast<unsigned long long>(b.dst[k]) & 15) != 0 ||
+            (OP != 1 && (reinterpret_cast<unsigned long long>(b.src[k]) & 15) != 0))
+            return hipErrorInvalidValue;
+    long long most = 0;
+    for (int k = 0; k < b.count; ++k) most = most > b.n[k] ? most : b.n[k];
+    if (most == 0) return hipSuccess;
+    const long long blocks = (most / 4 + 255) / 256;
+    const unsigned gx = unsigned(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+    hipLaunchKernelGGL(array_kernel<OP>, dim3(gx, unsigned(b.count)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+dim3 updated_grid(const DevGrid& g) {
+    return dim3(unsigned((g.gx - 8 + 63) / 64), unsigned((g.gy - 8 + 3) / 4), unsigned(g.gz - 8));
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_stress_local, updated_grid(P.g), dim3(64, 4), 0, s, P);
+    hipLaunchKernelGGL(adjoint_stress_gather, updated_grid(P.g), dim3(64, 4), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_velocity_local, updated_grid(P.g), dim3(64, 4), 0, s, P);
+    hipLaunchKernelGGL(adjoint_velocity_gather, updated_grid(P.g), dim3(64, 4), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_inject(const AdjStep& P, const int* rec, int count, const float* residual, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_inject, dim3((3 * count + 255) / 256), dim3(256), 0, s, P, rec, count, residual);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const float* gather, const float* observed, float* residual, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(residual_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gather, observed, residual, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_cells(const AdjStep& P, float* g_ib, float* out_mu, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_cells, dim3(unsigned((P.g.gx + 63) / 64), unsigned(P.g.gy), unsigned(P.g.gz)), dim3(64),
+                       0, s, P, g_ib, out_mu);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_fold(const FoldParams& F, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_fold, dim3(unsigned((F.nx + 63) / 64), unsigned(F.ny), unsigned(F.nz)), dim3(64), 0, s, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_arrays(const ArrayBatch& b, hipStream_t s) { return launch_arrays<0>(b, s); }
+hipError_t launch_zero_arrays(const ArrayBatch& b, hipStream_t s) { return launch_arrays<1>(b, s); }
+hipError_t launch_add_arrays(const ArrayBatch& b, hipStream_t s) { return launch_arrays<2>(b, s); }
+
+}  // namespace seis3d

@@ -20,49 +20,10 @@
 #include <vector>
 
 #include "../../include/seis3d.h"
+#include "seis3d_host.hpp"
 #include "seis3d_kernels.hpp"
 
-namespace {
-
-constexpr int kRadius = 4;
-constexpr std::size_t kMaxPhysicalNodes = std::size_t(300) * 300 * 400;   // Common3D.h:10
-constexpr int kMaxVolumeAxis = 256;                                        // Common3D.h:11
-constexpr int kMaxVolumeOutput = 1536;                                     // Common3D.h:12
-constexpr int kMaxReceivers = 300;                                         // Common3D.h:14
-constexpr int kMaxSegySamples = 32767;                                     // Common3D.h:15
-constexpr int kMaxSegyIntervalUs = 32767;                                  // Common3D.h:16
-constexpr float kSourceNorm = 10.8387785335f;                              // (1 + 2 exp(-1/2))^3
-
-thread_local std::string g_error;
-
-struct Failure {
-    int code;
-    std::string message;
-};
-
-[[noreturn]] void fail(int code, const std::string& message) { throw Failure{code, message}; }
-[[noreturn]] void invalid(const std::string& message) { fail(SEIS3D_EINVAL, message); }
-
-void check(hipError_t e, const char* operation) {
-    if (e != hipSuccess) fail(SEIS3D_EHIP, std::string("Seis3D / ") + operation + ": " + hipGetErrorString(e));
-}
-
-template <class F>
-int guarded(F&& fn) {
-    try {
-        fn();
-        return SEIS3D_OK;
-    } catch (const Failure& f) {
-        g_error = f.message;
-        return f.code;
-    } catch (const std::bad_alloc&) {
-        g_error = "Seis3D: host allocation failed.";
-        return SEIS3D_EHIP;
-    } catch (const std::exception& e) {
-        g_error = e.what();
-        return SEIS3D_EHIP;
-    }
-}
+namespace seis3d::host {
 
 // validMaterial3D (cuda/Cuda_Seis3D.cu:30-34).
 bool valid_material(float vp, float vs, float rho) {
@@ -70,7 +31,6 @@ bool valid_material(float vp, float vs, float rho) {
            vs < vp && vp * vp > (4.0f / 3.0f) * vs * vs && rho > 0 && rho <= 50000;
 }
 
-int active_pml(const seis3d_ctx& c) { return c.reflecting ? 0 : c.pml; }
 
 // validateContext3D (cuda/Cuda_Seis3D.cu:433-483).
 void validate(seis3d_ctx& c) {
@@ -181,15 +141,7 @@ float ricker(float frequency, double time) {
     return float((1 - 2 * phase * phase) * std::exp(-phase * phase));
 }
 
-std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
 
-// Device allocation plan for one context.
-struct Plan {
-    int px = 0;
-    std::size_t plane = 0, cells = 0;   // pitched cells
-    std::size_t slab[3]{};              // floats per CPML slot
-    std::size_t bytes = 0;
-};
 Plan plan_for(const seis3d_ctx& c) {
     Plan p;
     p.px = int(align_up(std::size_t(c.gx), 64));
@@ -207,82 +159,10 @@ Plan plan_for(const seis3d_ctx& c) {
     return p;
 }
 
-}  // namespace
+}  // namespace seis3d::host
 
-// ---------------------------------------------------------------------------
-// Session
-// ---------------------------------------------------------------------------
 
-struct seis3d_session {
-    seis3d_ctx c{};
-    Plan plan;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    float* fields[9]{};
-    float* inv_rho = nullptr;
-    float* lambda = nullptr;
-    float* mu = nullptr;
-    float* inv_mu = nullptr;           // 1/mu, rebuilt before the next step after a model change
-    bool inv_mu_dirty = true;
-    float* psi[3]{};
-    float* coef[3]{};
-    unsigned int* flags = nullptr;
-    int* receivers = nullptr;          // padded coordinates, 3 ints each
-    float* receiver_values = nullptr;
-    float* host_values = nullptr;      // pinned
-    int receiver_count = 0;
-    float* gather = nullptr;           // [sample][receiver][3]
-    int gather_target = 0, gather_count = 0, gather_stride = 1, gather_receivers = 0;
-    bool recording = false;
-    bool ghosts_valid = true;          // σ ghost planes consistent with the interior
-    seis3d::KernelConfig kernels;
-
-    seis3d::DevStep base() const {
-        seis3d::DevStep P{};
-        P.g.gx = c.gx; P.g.gy = c.gy; P.g.gz = c.gz;
-        P.g.px = plan.px;
-        P.g.plane = static_cast<long long>(plan.plane);
-        P.g.pml = active_pml(c);
-        P.g.free_surface = c.free_surface;
-        P.g.tz = (c.free_surface ? 1 : 2) * P.g.pml;
-        for (int a = 0; a < 3; ++a) P.g.slab[a] = static_cast<long long>(plan.slab[a]);
-        for (int f = 0; f < 9; ++f) P.p.f[f] = fields[f];
-        P.p.inv_rho = inv_rho; P.p.lambda = lambda; P.p.mu = mu; P.p.inv_mu = inv_mu;
-        for (int a = 0; a < 3; ++a) { P.p.psi[a] = psi[a]; P.p.coef[a] = coef[a]; }
-        P.p.flags = flags;
-        P.dt = c.dt;
-        P.invh = 1.0f / c.spacing;
-        P.src.active = 0;
-        return P;
-    }
-
-    seis3d::MaterialParams material_params() const {
-        seis3d::MaterialParams m{};
-        m.nx = c.nx; m.ny = c.ny; m.nz = c.nz; m.ox = c.ox; m.oy = c.oy; m.oz = c.oz;
-        m.kind = c.model_kind;
-        m.vp = c.vp; m.vs = c.vs; m.rho = c.density;
-        m.layer_vp = c.layer_vp; m.layer_vs = c.layer_vs; m.layer_rho = c.layer_density;
-        m.interface_depth = c.interface_depth; m.layer_dip = c.layer_dip; m.layer_thickness = c.layer_thickness;
-        m.fault_throw = c.fault_throw; m.anomaly_radius = c.anomaly_radius;
-        return m;
-    }
-
-    void release() noexcept {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& p : fields) { (void)hipFree(p); p = nullptr; }
-        (void)hipFree(inv_rho); (void)hipFree(lambda); (void)hipFree(mu); (void)hipFree(inv_mu);
-        inv_rho = lambda = mu = inv_mu = nullptr;
-        for (int a = 0; a < 3; ++a) { (void)hipFree(psi[a]); (void)hipFree(coef[a]); psi[a] = coef[a] = nullptr; }
-        (void)hipFree(flags); (void)hipFree(receivers); (void)hipFree(receiver_values); (void)hipFree(gather);
-        flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr;
-        if (host_values) (void)hipHostFree(host_values);
-        host_values = nullptr;
-        if (stream) (void)hipStreamDestroy(stream);
-        stream = nullptr;
-    }
-};
-
-namespace {
+namespace seis3d::host {
 
 void require_session(const seis3d_t* t) {
     if (!t || !t->stream) fail(SEIS3D_ELIFECYCLE, "Seis3D: simulation not initialized.");
@@ -334,7 +214,7 @@ void default_tiles(seis3d_t* t) {
 
 // One runStep3D (cuda/Cuda_Seis3D.cu:602-624), enqueued on the session stream.
 // Optional events bracket the two fused half-steps.
-void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev = nullptr) {
+void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev) {
     const seis3d_ctx& c = t->c;
     seis3d::DevStep P = t->base();
     if (t->inv_mu_dirty) {
@@ -375,7 +255,9 @@ void read_flag(seis3d_t* t, unsigned int* flag) {
           "check capture stability");
 }
 
-}  // namespace
+}  // namespace seis3d::host
+
+using namespace seis3d::host;
 
 extern "C" {
 

@@ -1,0 +1,161 @@
+// seis3d_host.hpp - internal host-side declarations shared by the C ABI
+// (seis3d_api.cpp) and the adjoint / FWI driver (seis3d_fwi.cpp): error
+// plumbing, the device allocation plan and the session object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/seis3d.h"
+#include "seis3d_kernels.hpp"
+
+namespace seis3d::host {
+
+
+inline constexpr int kRadius = 4;
+inline constexpr std::size_t kMaxPhysicalNodes = std::size_t(300) * 300 * 400;   // Common3D.h:10
+inline constexpr int kMaxVolumeAxis = 256;                                        // Common3D.h:11
+inline constexpr int kMaxVolumeOutput = 1536;                                     // Common3D.h:12
+inline constexpr int kMaxReceivers = 300;                                         // Common3D.h:14
+inline constexpr int kMaxSegySamples = 32767;                                     // Common3D.h:15
+inline constexpr int kMaxSegyIntervalUs = 32767;                                  // Common3D.h:16
+inline constexpr float kSourceNorm = 10.8387785335f;                              // (1 + 2 exp(-1/2))^3
+
+inline thread_local std::string g_error;
+
+struct Failure {
+    int code;
+    std::string message;
+};
+
+[[noreturn]] inline void fail(int code, const std::string& message) { throw Failure{code, message}; }
+[[noreturn]] inline void invalid(const std::string& message) { fail(SEIS3D_EINVAL, message); }
+
+inline void check(hipError_t e, const char* operation) {
+    if (e != hipSuccess) fail(SEIS3D_EHIP, std::string("Seis3D / ") + operation + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return SEIS3D_OK;
+    } catch (const Failure& f) {
+        g_error = f.message;
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "Seis3D: host allocation failed.";
+        return SEIS3D_EHIP;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SEIS3D_EHIP;
+    }
+}
+
+inline int active_pml(const seis3d_ctx& c) { return c.reflecting ? 0 : c.pml; }
+inline std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
+
+// Device allocation plan for one context.
+struct Plan {
+    int px = 0;
+    std::size_t plane = 0, cells = 0;   // pitched cells
+    std::size_t slab[3]{};              // floats per CPML slot
+    std::size_t bytes = 0;
+};
+
+Plan plan_for(const seis3d_ctx& c);
+std::vector<float> make_coefficients(const seis3d_ctx& c, int axis);
+float ricker(float frequency, double time);
+
+}  // namespace seis3d::host
+
+// ---------------------------------------------------------------------------
+// Session
+// ---------------------------------------------------------------------------
+
+struct seis3d_session {
+    using Plan = seis3d::host::Plan;
+    seis3d_ctx c{};
+    Plan plan;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* fields[9]{};
+    float* inv_rho = nullptr;
+    float* lambda = nullptr;
+    float* mu = nullptr;
+    float* inv_mu = nullptr;           // 1/mu, rebuilt before the next step after a model change
+    bool inv_mu_dirty = true;
+    float* psi[3]{};
+    float* coef[3]{};
+    unsigned int* flags = nullptr;
+    int* receivers = nullptr;          // padded coordinates, 3 ints each
+    float* receiver_values = nullptr;
+    float* host_values = nullptr;      // pinned
+    int receiver_count = 0;
+    float* gather = nullptr;           // [sample][receiver][3]
+    int gather_target = 0, gather_count = 0, gather_stride = 1, gather_receivers = 0;
+    bool recording = false;
+    bool ghosts_valid = true;          // σ ghost planes consistent with the interior
+    seis3d::KernelConfig kernels;
+
+    seis3d::DevStep base() const {
+        seis3d::DevStep P{};
+        P.g.gx = c.gx; P.g.gy = c.gy; P.g.gz = c.gz;
+        P.g.px = plan.px;
+        P.g.plane = static_cast<long long>(plan.plane);
+        P.g.pml = seis3d::host::active_pml(c);
+        P.g.free_surface = c.free_surface;
+        P.g.tz = (c.free_surface ? 1 : 2) * P.g.pml;
+        for (int a = 0; a < 3; ++a) P.g.slab[a] = static_cast<long long>(plan.slab[a]);
+        for (int f = 0; f < 9; ++f) P.p.f[f] = fields[f];
+        P.p.inv_rho = inv_rho; P.p.lambda = lambda; P.p.mu = mu; P.p.inv_mu = inv_mu;
+        for (int a = 0; a < 3; ++a) { P.p.psi[a] = psi[a]; P.p.coef[a] = coef[a]; }
+        P.p.flags = flags;
+        P.dt = c.dt;
+        P.invh = 1.0f / c.spacing;
+        P.src.active = 0;
+        return P;
+    }
+
+    seis3d::MaterialParams material_params() const {
+        seis3d::MaterialParams m{};
+        m.nx = c.nx; m.ny = c.ny; m.nz = c.nz; m.ox = c.ox; m.oy = c.oy; m.oz = c.oz;
+        m.kind = c.model_kind;
+        m.vp = c.vp; m.vs = c.vs; m.rho = c.density;
+        m.layer_vp = c.layer_vp; m.layer_vs = c.layer_vs; m.layer_rho = c.layer_density;
+        m.interface_depth = c.interface_depth; m.layer_dip = c.layer_dip; m.layer_thickness = c.layer_thickness;
+        m.fault_throw = c.fault_throw; m.anomaly_radius = c.anomaly_radius;
+        return m;
+    }
+
+    void release() noexcept {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& p : fields) { (void)hipFree(p); p = nullptr; }
+        (void)hipFree(inv_rho); (void)hipFree(lambda); (void)hipFree(mu); (void)hipFree(inv_mu);
+        inv_rho = lambda = mu = inv_mu = nullptr;
+        for (int a = 0; a < 3; ++a) { (void)hipFree(psi[a]); (void)hipFree(coef[a]); psi[a] = coef[a] = nullptr; }
+        (void)hipFree(flags); (void)hipFree(receivers); (void)hipFree(receiver_values); (void)hipFree(gather);
+        flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr;
+        if (host_values) (void)hipHostFree(host_values);
+        host_values = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace seis3d::host {
+
+void require_session(const seis3d_t* t);
+void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what);
+void check_source(const seis3d_ctx& c, const seis3d_source& s);
+// Source term at time `time` (sourceKernel3D, cuda/Cuda_Seis3D.cu:313-332).
+seis3d::DevSource make_source(const seis3d_ctx& c, const seis3d_source& s, double time);
+// One runStep3D (cuda/Cuda_Seis3D.cu:602-624), enqueued on the session stream;
+// optional events bracket the two fused half-steps.
+void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev = nullptr);
+
+}  // namespace seis3d::host

@@ -95,6 +95,12 @@ class Survey(C.Structure):
                 ("shot_origin", C.c_int32 * 3), ("shot_spacing", C.c_int32 * 2)]
 
 
+class Shot(C.Structure):
+    """seis3d_shot: source, receivers and observed [samples][receivers][3] data of one FWI shot."""
+    _fields_ = [("source", Source), ("receivers", C.POINTER(Receiver)), ("receiver_count", C.c_int32),
+                ("observed", C.POINTER(C.c_float))]
+
+
 class Timing(C.Structure):
     _fields_ = [("velocity_ms", C.c_double), ("stress_ms", C.c_double), ("total_ms", C.c_double),
                 ("launches", C.c_int64)]
@@ -162,6 +168,10 @@ def library() -> C.CDLL:
             "seis3d_batch_launch": (C.c_int, [vp]),
             "seis3d_batch_synchronize": (C.c_int, [vp]),
             "seis3d_batch_destroy": (None, [vp]),
+            "seis3d_fwi_plan": (C.c_int, [P(Ctx), C.c_int64, C.c_int, C.c_int, C.c_int, P(C.c_uint64),
+                                          P(C.c_int)]),
+            "seis3d_fwi_gradient": (C.c_int, [vp, P(Shot), C.c_int, C.c_int64, C.c_int, C.c_int, P(C.c_double),
+                                              P(C.c_float), P(C.c_float), P(C.c_float)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -475,6 +485,31 @@ class GPUSimData3D:
         _check(library().seis3d_get_tiles(self._h, C.byref(f), C.byref(y), C.byref(z)))
         return f.value, y.value, z.value
 
+    def fwi_gradient(self, shots, n_steps: int, stride: int, checkpoint_interval: int = 0):
+        """Misfit and gradient ([nz, ny, nx] float32 vp, vs, rho) of shots (source, receivers, observed).
+
+        `observed` of each shot is [n_steps // stride, len(receivers), 3]; shots share this session's
+        model; all-CPML contexts only.
+        """
+        c = self._ctx_now()
+        keep = []
+        arr = (Shot * max(1, len(shots)))()
+        for k, (src, rec, obs) in enumerate(shots):
+            rec = list(rec)
+            r = (Receiver * len(rec))(*[Receiver(int(p[0]), int(p[1]), int(p[2])) for p in rec])
+            o = np.ascontiguousarray(obs, dtype=np.float32)
+            if o.shape != (n_steps // stride, len(rec), 3):
+                raise InvalidArgument(f"observed data must be {(n_steps // stride, len(rec), 3)}, got {o.shape}")
+            keep += [r, o]
+            arr[k] = Shot(src, r, len(rec), _fptr(o))
+        shape = (c.nz, c.ny, c.nx)
+        g = [np.zeros(shape, np.float32) for _ in range(3)]
+        m = C.c_double()
+        _check(library().seis3d_fwi_gradient(self._h, arr, len(shots), int(n_steps), int(stride),
+                                             int(checkpoint_interval), C.byref(m), *[_fptr(a) for a in g]))
+        del keep
+        return m.value, g[0], g[1], g[2]
+
 
 class Batch:
     """Several shots captured into one HIP graph (one launch runs every shot's steps).
@@ -507,6 +542,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def fwi_plan(ctx: Ctx, n_steps: int, stride: int, checkpoint_interval: int, receivers: int) -> dict:
+    """Device bytes per FWI shot and the checkpoint interval used."""
+    b, k = C.c_uint64(), C.c_int()
+    _check(library().seis3d_fwi_plan(C.byref(ctx), int(n_steps), int(stride), int(checkpoint_interval),
+                                     int(receivers), C.byref(b), C.byref(k)))
+    return {"shot_bytes": b.value, "interval": k.value}
 
 
 def device_count() -> int:

@@ -249,6 +249,35 @@ int seis3d_batch_launch(seis3d_batch_t* b);
 int seis3d_batch_synchronize(seis3d_batch_t* b);
 void seis3d_batch_destroy(seis3d_batch_t* b);
 
+/* ---- FWI gradient (no reference counterpart) -------------------------------- */
+
+/* One shot of a gradient request: its source, its receivers (physical
+ * coordinates, as seis3d_set_receivers) and the observed data, host memory
+ * laid out as the gather: [n_steps / stride][receiver_count][vx, vy, vz]. */
+typedef struct seis3d_shot {
+    seis3d_source source;
+    const seis3d_receiver* receivers;
+    int32_t receiver_count;
+    const float* observed;
+} seis3d_shot;
+
+/* Device bytes one shot of a gradient request needs, and the checkpoint
+ * interval used (checkpoint_interval <= 0 picks ceil(sqrt(n_steps))). */
+int seis3d_fwi_plan(const seis3d_ctx* ctx, int64_t n_steps, int stride, int checkpoint_interval,
+                    int receivers, uint64_t* shot_bytes, int* interval);
+
+/* Misfit 0.5 * sum over shots, samples, receivers and components of
+ * (synthetic - observed)^2 for n_steps steps sampled after steps stride,
+ * 2 stride, ... (as the gather), and its gradient with respect to vp, vs and
+ * rho at every physical cell, [nz][ny][nx] host arrays, by the discrete
+ * adjoint of the step.  The forward state is checkpointed every
+ * checkpoint_interval steps and recomputed segment by segment.  Shots share
+ * the session's model and run concurrently as device memory allows; the
+ * session's own wavefield, receivers and gather are left untouched.
+ * All-CPML contexts only (free surface and reflecting walls give EINVAL). */
+int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_t n_steps, int stride,
+                        int checkpoint_interval, double* misfit, float* g_vp, float* g_vs, float* g_rho);
+
 /* ---- Errors and build info ------------------------------------------------ */
 
 const char* seis3d_last_error(void);

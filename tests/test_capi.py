@@ -115,3 +115,31 @@ def test_lifecycle_errors_without_device():
         S._check(S.library().seis3d_reset(None))
     g.release()
     g.release()   # idempotent, as releaseGpu3D
+
+
+def test_fwi_plan_memory_formula():
+    """Per-shot FWI bytes: (1 live + checkpoints + K slots) states + adjoint state + records."""
+    c = W.imported_context(200, 100, 200, max_vp=4320.0, free_surface=False)
+    p = S.fwi_plan(c, 1200, 2, 50, 200)
+    assert p["interval"] == 50
+    px = (c.gx + 63) // 64 * 64
+    cells = px * c.gy * c.gz
+    pad = lambda n: (n + 63) // 64 * 64
+    psi = [6 * c.gz * c.gy * 40, 6 * c.gz * 40 * px, 6 * 40 * c.gy * px]
+    state = 9 * pad(cells) + sum(pad(x) for x in psi)
+    adjoint = 26 * pad(cells) + sum(pad(x) for x in psi)
+    records = 3 * pad(600 * 200 * 3)
+    expected = ((1 + 22 + 50) * state + adjoint + records) * 4 + pad(600) * 4 + 256
+    assert p["shot_bytes"] == expected
+    assert S.fwi_plan(c, 1200, 2, 0, 200)["interval"] == 35      # ceil(sqrt(1200))
+    assert S.fwi_plan(c, 30, 1, 100, 10)["interval"] == 30
+
+
+def test_fwi_plan_rejects_unsupported_requests():
+    free = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=True)
+    with pytest.raises(S.InvalidArgument, match="all-CPML"):
+        S.fwi_plan(free, 100, 1, 10, 5)
+    c = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=False)
+    for steps, stride, rec in ((0, 1, 5), (100, 0, 5), (100, 101, 5), (100, 1, 0), (100, 1, 301)):
+        with pytest.raises(S.InvalidArgument):
+            S.fwi_plan(c, steps, stride, 10, rec)
