@@ -42,9 +42,11 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", default="300x50x300")
-    p.add_argument("--workload", choices=["step", "batch"], default="step",
+    p.add_argument("--workload", choices=["step", "batch", "fwi"], default="step",
                    help="step: the fused 3D step (default line); batch: 16 uneven shots, graph vs plain loop")
-    p.add_argument("--shots", type=int, default=16)
+    p.add_argument("--shots", type=int, default=0, help="shots of the batch (16) or fwi (8) workload")
+    p.add_argument("--checkpoint", type=int, default=50, help="FWI checkpoint interval (steps)")
+    p.add_argument("--fd", type=int, default=0, help="FWI: seeded central-difference directions to check")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="instead of the bench, run 5 plain device copies of MIB MiB (counter calibration)")
     return p.parse_args()
@@ -89,7 +91,7 @@ def batch_bench(args) -> None:
     from cmakepro_cuda_seissim_v2_amd import workloads as W
 
     steps, stride = args.steps, 2
-    shots = W.batch_shots(args.shots)
+    shots = W.batch_shots(args.shots or 16)
     sessions = []
     for ctx, model, src, rec in shots:
         g = S.GPUSimData3D(ctx)
@@ -158,6 +160,111 @@ def batch_bench(args) -> None:
     print(json.dumps(out), flush=True)
 
 
+def fwi_misfit(S, ctx, model, shots, observed, steps, stride):
+    """0.5 |gather - observed|^2 over all shots from plain forward runs (FP32 GPU, FP64 sum)."""
+    import numpy as np
+    total = 0.0
+    with S.GPUSimData3D(ctx) as g:
+        g.upload_model(*model)
+        for (src, rec), obs in zip(shots, observed):
+            g.reset()
+            g.set_receivers(rec)
+            g.gather_begin(steps // stride, stride)
+            g.run(src, 0, steps)
+            got = g.gather_read()
+            g.gather_end()
+            total += 0.5 * float(np.sum((got.astype(np.float64) - obs.astype(np.float64)) ** 2))
+    return total
+
+
+def fwi_bench(args) -> None:
+    """FWI gradient of 8 shots on 200x100x200 (1200 steps, checkpoint every 50 steps).
+
+    batched: one seis3d_fwi_gradient request with every shot (shots on their own streams, as many
+    at once as fit in HBM). plain: the same library called once per shot, one after another.
+    Observed data come from forward runs on the true model (outside both timings).
+    """
+    import numpy as np
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    from cmakepro_cuda_seissim_v2_amd import workloads as W
+
+    steps, stride, interval = args.steps, 2, args.checkpoint
+    ctx, start, true, shots = W.fwi_workload(args.shots or 8)
+    plan = S.fwi_plan(ctx, steps, stride, interval, len(shots[0][1]))
+    print(f"bench: fwi {len(shots)} shots {ctx.nx}x{ctx.ny}x{ctx.nz}, {steps} steps, "
+          f"{plan['shot_bytes'] / 2**30:.1f} GiB per shot", file=sys.stderr, flush=True)
+    observed = []
+    with S.GPUSimData3D(ctx) as g:
+        g.upload_model(*true)
+        for src, rec in shots:
+            g.reset()
+            g.set_receivers(rec)
+            g.gather_begin(steps // stride, stride)
+            g.run(src, 0, steps)
+            observed.append(g.gather_read())
+            g.gather_end()
+    print("bench: observed data ready", file=sys.stderr, flush=True)
+    req = [(src, rec, obs) for (src, rec), obs in zip(shots, observed)]
+    with S.GPUSimData3D(ctx) as g:
+        g.upload_model(*start)
+        g.synchronize()
+        t0 = time.perf_counter()
+        batched = g.fwi_gradient(req, steps, stride, interval)
+        t_batch = time.perf_counter() - t0
+        print(f"bench: batched gradient {t_batch:.3f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        singles = [g.fwi_gradient([r], steps, stride, interval) for r in req]
+        t_plain = time.perf_counter() - t0
+        print(f"bench: plain per-shot loop {t_plain:.3f} s", file=sys.stderr, flush=True)
+    summed = [np.sum([s[k].astype(np.float64) for s in singles], axis=0) for k in (1, 2, 3)]
+    agree = max(float(np.linalg.norm(batched[k + 1] - summed[k]) / np.linalg.norm(summed[k])) for k in range(3))
+    n_upd = (ctx.gx - 8) * (ctx.gy - 8) * (ctx.gz - 8)
+    fd = []
+    if args.fd:
+        base = [np.asarray(a, np.float32) for a in start]
+        for j in range(args.fd):
+            seed = 700 + j
+            mask = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0), (1, 1, 1)][j % 5]
+            d = [W.gaussian_random_field(base[0].shape, 15.0, seed + 100 * k).astype(np.float64)
+                 * base[k] * mask[k] for k in range(3)]
+            ratios = []
+            for eps in (4e-3, 8e-3):
+                js, moved = [], []
+                for sign in (1, -1):
+                    m = [(b.astype(np.float64) + sign * eps * dk).astype(np.float32) for b, dk in zip(base, d)]
+                    moved.append(m)
+                    js.append(fwi_misfit(S, ctx, tuple(m), shots, observed, steps, stride))
+                pred = sum(float(np.sum(gk.astype(np.float64) * (p.astype(np.float64) - q.astype(np.float64))))
+                           for gk, p, q in zip(batched[1:], moved[0], moved[1]))
+                ratios.append((js[0] - js[1]) / pred)
+            extrap = (4 * ratios[0] - ratios[1]) / 3
+            fd.append({"direction": ["vp", "vs", "rho", "vp", "vp+vs+rho"][j % 5], "seed": seed,
+                       "fd_over_adjoint": [round(r, 7) for r in ratios], "extrapolated": round(extrap, 7),
+                       "rel_error": float(f"{abs(extrap - 1):.3e}")})
+            print(f"bench: fd direction {fd[-1]}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "FWI gradient shots/s (vp, vs, rho; discrete adjoint + checkpoints), batched vs plain per-shot loop",
+        "value": round(len(shots) / t_batch, 4),
+        "unit": "shots/s",
+        "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(t_batch / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (smooth gradient background + seeded Gaussian anomaly, +8 % vp, radius 15 cells)",
+        "config": {
+            "workload": f"FWI gradient, {len(shots)} shots, {ctx.nx}x{ctx.ny}x{ctx.nz}, dx=10 m, all-CPML 20 cells, "
+                        f"{steps} steps, 15 Hz explosions, {len(shots[0][1])} receivers stride {stride}, "
+                        f"checkpoint every {interval} steps",
+            "dt_us": round(ctx.dt * 1e6), "n_upd": n_upd, "shot_bytes": plan["shot_bytes"],
+            "plain_shots_per_s": round(len(shots) / t_plain, 4),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "batched_vs_plain_rel_l2": float(f"{agree:.3e}"),
+            "misfit": batched[0],
+            "fd_checks": fd,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_info():
     cores = os.cpu_count() or 1
     try:
@@ -202,6 +309,10 @@ def cpu_baseline(ctx, model, src, rec, steps_hint):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -210,10 +321,11 @@ def main():
             args.steps = 800
         batch_bench(args)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
-        sys.exit(2)
+    if args.workload == "fwi":
+        if args.steps == 1000:
+            args.steps = 1200
+        fwi_bench(args)
+        return
 
     from cmakepro_cuda_seissim_v2_amd import seis3d as S
     from cmakepro_cuda_seissim_v2_amd import workloads as W

@@ -389,7 +389,10 @@ int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_
         if (free_bytes < totals_bytes + f.shot_bytes)
             invalid("Seis3D: the FWI checkpoint plan does not fit in device memory.");
         const std::size_t usable = std::size_t(double(free_bytes - totals_bytes) * 0.94);
-        const int wave = int(std::max<std::size_t>(1, std::min<std::size_t>(std::size_t(count), usable / f.shot_bytes)));
+        const int fit = int(std::max<std::size_t>(1, std::min<std::size_t>(std::size_t(count), usable / f.shot_bytes)));
+        // Equal waves (8 shots with room for 7 run as 4 + 4, not 7 + 1).
+        const int waves = (count + fit - 1) / fit;
+        const int wave = (count + waves - 1) / waves;
 
         float* totals_mem = nullptr;
         std::vector<ShotWork> work(static_cast<std::size_t>(wave));

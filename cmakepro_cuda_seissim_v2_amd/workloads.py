@@ -107,3 +107,36 @@ def batch_shots(count: int = 16, seed: int = BASE_SEED, sizes=None, corr: float 
         rec = [(int(round(i * (nx - 1) / max(1, nrec - 1))), ny // 2, 0) for i in range(nrec)]
         shots.append((ctx, (vp, vs, rho), src, rec))
     return shots
+
+
+def fwi_workload(count: int = 8, nx: int = 200, ny: int = 100, nz: int = 200, seed: int = BASE_SEED,
+                 radius: float = 15.0, gain: float = 0.08, receivers: int = 200, pml: int = 20,
+                 frequency: float = 15.0):
+    """FWI gradient workload: (context, start model, true model, [(source, receivers)] per shot).
+
+    Background: smooth depth gradient vp 2000 -> 4000 m/s, vs = vp/1.73, Gardner density (the
+    starting model). True model: the background with vp * (1 + gain * exp(-(r/radius)^2)) around
+    a seeded centre in the middle third of the model. All-CPML boundaries (the adjoint supports
+    no free surface). Shots: `count` 15 Hz explosions at depth 5 spread evenly along x at y = ny/2;
+    each records `receivers` receivers on a line along x at depth 2 through y = ny/2.
+    """
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz, dtype=np.float32) / max(1, nz - 1)
+    vp = np.broadcast_to((2000.0 + 2000.0 * z)[:, None, None], (nz, ny, nx)).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    start = (vp, vs, gardner_density(vp))
+    cx, cy, cz = (float(rng.uniform(n / 3, 2 * n / 3)) for n in (nx, ny, nz))
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = ((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / radius ** 2
+    vp_true = (vp * (1.0 + gain * np.exp(-r2))).astype(np.float32)
+    true = (vp_true, vs, gardner_density(vp))
+    ctx = imported_context(nx, ny, nz, max_vp=float(vp_true.max()), pml=pml, free_surface=False,
+                           frequency=frequency)
+    nrec = min(receivers, nx)
+    rec = [(int(round(i * (nx - 1) / max(1, nrec - 1))), ny // 2, 2) for i in range(nrec)]
+    shots = []
+    for k in range(count):
+        sx = int(round(nx * (k + 0.5) / count))
+        shots.append((S.default_source(x=min(max(sx, 1), nx - 2), y=ny // 2, z=5, kind=S.EXPLOSION,
+                                       amplitude=1.0e8), rec))
+    return ctx, start, true, shots
