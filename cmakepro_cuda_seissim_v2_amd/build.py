@@ -32,6 +32,7 @@ HIP_FLAGS = [
 ]
 
 LIBRARIES = {
+    "libseis2d.so": ["seis2d_kernels.hip", "seis2d_api.cpp"],
     "libseis3d.so": ["seis3d_kernels.hip", "seis3d_pipe.hip", "seis3d_column.hip", "seis3d_dma.hip", "seis3d_adjoint.hip",
                     "seis3d_api.cpp", "seis3d_fwi.cpp"],
 }
@@ -68,7 +69,7 @@ def build_oracle() -> Path:
     out = out_dir / "liboracle3d.so"
     tmp = out.with_suffix(".so.tmp")
     _run([CXX, "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-std=c++17", "-fPIC", "-shared",
-          "-Wall", str(ORACLE_DIR / "oracle_capi.cpp"), "-o", str(tmp)])
+          "-Wall", str(ORACLE_DIR / "oracle_capi.cpp"), str(ORACLE_DIR / "oracle2d_capi.cpp"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 
