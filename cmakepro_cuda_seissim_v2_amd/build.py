@@ -74,19 +74,19 @@ def build_oracle() -> Path:
     return out
 
 
-CPP_TESTS = ["seis3d_boundary_regression"]
+CPP_TESTS = {"seis3d_boundary_regression": "seis3d", "seis2d_solver_regression": "seis2d"}
 
 
 def build_cpp_tests() -> list[Path]:
-    """Compile the C++ test programs (tests/cpp) against include/Seis3D.hpp and libseis3d.so."""
+    """Compile the C++ test programs (tests/cpp) against include/Seis3D.hpp / Seis2D.hpp and their libraries."""
     src_dir = REPO_DIR / "tests" / "cpp"
     out_dir = src_dir / "bin"
     out_dir.mkdir(parents=True, exist_ok=True)
     outs = []
-    for name in CPP_TESTS:
+    for name, lib in CPP_TESTS.items():
         out = out_dir / name
         _run([CXX, "-O2", "-std=c++17", "-Wall", f"-I{REPO_DIR / 'include'}", str(src_dir / f"{name}.cpp"),
-              "-o", str(out), f"-L{LIB_DIR}", "-lseis3d", f"-Wl,-rpath,$ORIGIN/../../../cmakepro_cuda_seissim_v2_amd/lib"])
+              "-o", str(out), f"-L{LIB_DIR}", f"-l{lib}", f"-Wl,-rpath,$ORIGIN/../../../cmakepro_cuda_seissim_v2_amd/lib"])
         outs.append(out)
     return outs
 

@@ -31,3 +31,16 @@ def test_boundary_regression_gpu(gpu):
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stderr + r.stdout[-2000:]
     assert "PASS" in r.stdout
+
+
+def test_2d_solver_regression_cpu_checks():
+    r = run("seis2d_solver_regression", 600)
+    assert r.returncode in (0, 77), r.stderr + r.stdout[-2000:]
+
+
+@pytest.mark.gpu
+def test_2d_solver_regression_gpu(gpu):
+    r = run("seis2d_solver_regression", 600)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert "PASS" in r.stdout
