@@ -34,7 +34,7 @@ HIP_FLAGS = [
 LIBRARIES = {
     "libseis2d.so": ["seis2d_kernels.hip", "seis2d_api.cpp"],
     "libseis3d.so": ["seis3d_kernels.hip", "seis3d_pipe.hip", "seis3d_column.hip", "seis3d_dma.hip", "seis3d_adjoint.hip",
-                    "seis3d_api.cpp", "seis3d_fwi.cpp"],
+                    "seis3d_abc.hip", "seis3d_api.cpp", "seis3d_fwi.cpp", "seis3d_abc.cpp", "seis3d_io.cpp"],
 }
 
 
@@ -50,7 +50,7 @@ def build_library(name: str = "libseis3d.so") -> Path:
     out = LIB_DIR / name
     objs = []
     for src in LIBRARIES[name]:
-        obj = LIB_DIR / (Path(src).stem + ".o")
+        obj = LIB_DIR / (src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, "-c", *lang, str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))

@@ -1,0 +1,561 @@
+// seis3d_abc.hip - learned absorbing boundary kernels (no reference
+// counterpart): frame-cell features, the BF16 MFMA MLP (inference and
+// training forward), FP32 backward, parameter-gradient reductions and Adam.
+//
+// MFMA: v_mfma_f32_16x16x16_bf16 on 64-lane wavefronts.  Lane l supplies
+// A[row l%16][k 4(l/16)+j] and B[k 4(l/16)+j][col l%16] (j = 0..3) and holds
+// D[row 4(l/16)+j][col l%16].  A wavefront evaluates 16 cells per tile:
+// 6 + 2 MFMAs per 16-column block of the 96->32 and 32->32 layers and 2 for
+// the 32->9 layer.  Weights (9.2 KB BF16) stay in LDS for the whole grid
+// stride loop.
+#include "seis3d_abc.hpp"
+
+namespace seis3d::abc {
+
+namespace {
+
+typedef short __attribute__((ext_vector_type(4))) bf16x4;
+typedef float __attribute__((ext_vector_type(4))) floatx4;
+
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+
+// Round-to-nearest-even FP32 -> BF16 (finite inputs).
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+    const unsigned int u = __float_as_uint(f);
+    return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
+
+// Face-frame component -> global field, per normal axis (normal component first).
+__constant__ int kFaceField[3][9] = {{0, 1, 2, 3, 4, 5, 6, 7, 8}, {1, 2, 0, 4, 5, 3, 7, 8, 6}, {2, 0, 1, 5, 3, 4, 8, 6, 7}};
+// Components odd in the normal coordinate (v_n, s_nt1, s_nt2) flip sign on low faces.
+__constant__ float kOdd[9] = {1, 0, 0, 0, 0, 0, 1, 0, 1};
+
+struct Lds {
+    unsigned short w1[kHidden * kIn];
+    unsigned short w2[kHidden * kHidden];
+    unsigned short w3[kOutPad * kHidden];
+    float b1[kHidden], b2[kHidden], b3[kOutPad];
+    float xf[kWaves][16][kIn];
+    unsigned short xs[kWaves][16][kIn];
+    unsigned short h1[kWaves][16][kHidden];
+    unsigned short h2[kWaves][16][kHidden];
+    float scale[kWaves][16];
+};
+
+__device__ void load_image(Lds& L, const void* image) {
+    const unsigned short* w = static_cast<const unsigned short*>(image);
+    for (int k = threadIdx.x; k < kHidden * kIn; k += kBlock) L.w1[k] = w[k];
+    for (int k = threadIdx.x; k < kHidden * kHidden; k += kBlock) L.w2[k] = w[kHidden * kIn + k];
+    for (int k = threadIdx.x; k < kOutPad * kHidden; k += kBlock) L.w3[k] = w[kHidden * kIn + kHidden * kHidden + k];
+    const float* b = reinterpret_cast<const float*>(w + kBf16Count);
+    for (int k = threadIdx.x; k < kHidden; k += kBlock) { L.b1[k] = b[k]; L.b2[k] = b[kHidden + k]; }
+    for (int k = threadIdx.x; k < kOutPad; k += kBlock) L.b3[k] = b[2 * kHidden + k];
+}
+
+// MLP of one 16-cell tile held in L.xs[wave]; returns lane's layer-3 outputs
+// (rows 4q+j, column lane%16).  Optionally stores BF16-rounded activations.
+__device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long row0, int rows) {
+    const int lane = int(threadIdx.x) & 63, m = lane & 15, q = lane >> 4;
+    floatx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kb = 0; kb < kIn / 16; ++kb) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.xs[wave][m][kb * 16 + 4 * q]);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w1[(nb * 16 + m) * kIn + kb * 16 + 4 * q]);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned short h = to_bf16(fmaxf(acc[nb][j] + L.b1[nb * 16 + m], 0.0f));
+            L.h1[wave][4 * q + j][nb * 16 + m] = h;
+            if (H1 && 4 * q + j < rows) H1[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
+        }
+    __syncthreads();
+    floatx4 acc2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int kb = 0; kb < kHidden / 16; ++kb) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.h1[wave][m][kb * 16 + 4 * q]);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w2[(nb * 16 + m) * kHidden + kb * 16 + 4 * q]);
+            acc2[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc2[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned short h = to_bf16(fmaxf(acc2[nb][j] + L.b2[nb * 16 + m], 0.0f));
+            L.h2[wave][4 * q + j][nb * 16 + m] = h;
+            if (H2 && 4 * q + j < rows) H2[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
+        }
+    __syncthreads();
+    floatx4 out = {0, 0, 0, 0};
+#pragma unroll
+    for (int kb = 0; kb < kHidden / 16; ++kb) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.h2[wave][m][kb * 16 + 4 * q]);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w3[m * kHidden + kb * 16 + 4 * q]);
+        out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, out, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] += L.b3[m];
+    return out;
+}
+
+__device__ __forceinline__ long long stride_of(const DevGrid& g, int axis) {
+    return axis == 0 ? 1LL : axis == 1 ? static_cast<long long>(g.px) : g.plane;
+}
+__device__ __forceinline__ int coord_of(const DevGrid& g, long long i, int axis) {
+    if (axis == 0) return int(i % g.px);
+    if (axis == 1) return int((i / g.px) % g.gy);
+    return int(i / g.plane);
+}
+__device__ __forceinline__ int extent_of(const DevGrid& g, int axis) { return axis == 0 ? g.gx : axis == 1 ? g.gy : g.gz; }
+
+__device__ __forceinline__ float impedance(const Frame& F, long long i) {
+    return sqrtf((F.lambda[i] + 2.0f * F.mu[i]) / F.inv_rho[i]);
+}
+
+// The 96 raw (un-normalised) features of feature indices part, part+4, ... of cell i into xf row.
+__device__ void gather_row(const Frame& F, int i, unsigned char code, int part, float* row) {
+    const int axis = code >> 1;
+    const bool hi = (code & 1) != 0;
+    const long long st = stride_of(F.g, axis) * (hi ? 1 : -1);
+    const int c0 = coord_of(F.g, i, axis), n = extent_of(F.g, axis);
+    const float z = impedance(F, i);
+    for (int f = part; f < kIn; f += 4) {
+        const int ch = f / kTaps, tap = f % kTaps, o = tap - 3;
+        const int c = c0 + (hi ? o : -o);
+        const long long p = i + o * st;
+        const bool inside = c >= 4 && c < n - 4;
+        float v;
+        if (ch < 9) {
+            v = F.f[kFaceField[axis][ch]][p];
+            if (!hi && kOdd[ch] != 0.0f) v = -v;
+            if (ch >= 3) v = v / z;
+        } else if (ch == 9) {
+            v = inside ? 1.0f : 0.0f;
+        } else if (ch == 10) {
+            v = sqrtf((F.lambda[p] + 2.0f * F.mu[p]) * F.inv_rho[p]) * (1.0f / 5000.0f);
+        } else {
+            v = inside ? (hi ? 1.0f : -1.0f) : 0.0f;
+        }
+        row[f] = v;
+    }
+}
+
+// Scale s = max |field feature| of each tile row; rows normalised into xs (BF16) / X (FP32).
+__device__ void normalise_tile(Lds& L, int wave, int rows, float* X, float* S, long long row0) {
+    const int lane = int(threadIdx.x) & 63, cellr = lane >> 2, part = lane & 3;
+    float mx = 0.0f;
+    for (int f = part; f < 9 * kTaps; f += 4) mx = fmaxf(mx, fabsf(L.xf[wave][cellr][f]));
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 2));
+    const float inv = mx > 0.0f ? 1.0f / mx : 0.0f;
+    for (int f = part; f < kIn; f += 4) {
+        const float v = f < 9 * kTaps ? L.xf[wave][cellr][f] * inv : (mx > 0.0f ? L.xf[wave][cellr][f] : 0.0f);
+        L.xs[wave][cellr][f] = to_bf16(v);
+        if (X && cellr < rows) X[(row0 + cellr) * kIn + f] = v;
+    }
+    if (part == 0) {
+        L.scale[wave][cellr] = mx;
+        if (S && cellr < rows) S[row0 + cellr] = mx;
+    }
+}
+
+// Inference: corrections of all frame cells into delta [count][9] (global field order).
+__global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, const void* image, float* delta) {
+    extern __shared__ unsigned char smem[];
+    Lds& L = *reinterpret_cast<Lds*>(smem);
+    load_image(L, image);
+    __syncthreads();
+    const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const int tiles = (C.count + 15) / 16;
+    const int waves_total = int(gridDim.x) * kWaves;
+    const int iters = (tiles + waves_total - 1) / waves_total;
+    for (int it = 0; it < iters; ++it) {
+        const int tile = it * waves_total + int(blockIdx.x) * kWaves + wave;
+        const int first = tile * 16;
+        const int rows = tile < tiles ? min(16, C.count - first) : 0;
+        const int cellr = lane >> 2, part = lane & 3;
+        if (cellr < rows) gather_row(F, C.cell[first + cellr], C.code[first + cellr], part, L.xf[wave][cellr]);
+        else
+            for (int f = part; f < kIn; f += 4) L.xf[wave][cellr][f] = 0.0f;
+        __syncthreads();
+        normalise_tile(L, wave, rows, nullptr, nullptr, 0);
+        __syncthreads();
+        const floatx4 out = mlp_tile(L, wave, nullptr, nullptr, 0, 0);
+        const int m = lane & 15, q = lane >> 4;
+        if (m < kOut) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = 4 * q + j;
+                if (r >= rows) continue;
+                const int i = C.cell[first + r];
+                const unsigned char code = C.code[first + r];
+                const int axis = code >> 1;
+                const bool hi = (code & 1) != 0;
+                float v = out[j] * L.scale[wave][r];
+                if (m >= 3) v *= impedance(F, i);
+                if (!hi && kOdd[m] != 0.0f) v = -v;
+                delta[static_cast<long long>(first + r) * kOut + kFaceField[axis][m]] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+struct FieldPtrs {
+    float* f[9];
+};
+__global__ void apply_kernel_v(FieldPtrs P, FrameCells C, const float* delta) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= C.count) return;
+    const int i = C.cell[k];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) P.f[c][i] += delta[static_cast<long long>(k) * kOut + c];
+}
+
+// Features, scales and face-frame scaled values of the listed cells (one thread per cell).
+__global__ __launch_bounds__(128) void features_kernel(Frame F, FrameCells C, float* X, float* S, float* B) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= C.count) return;
+    float row[kIn];
+    const int i = C.cell[k];
+    const unsigned char code = C.code[k];
+    for (int p = 0; p < 4; ++p) gather_row(F, i, code, p, row);
+    float mx = 0.0f;
+    for (int f = 0; f < 9 * kTaps; ++f) mx = fmaxf(mx, fabsf(row[f]));
+    const float inv = mx > 0.0f ? 1.0f / mx : 0.0f;
+    for (int f = 0; f < kIn; ++f)
+        X[static_cast<long long>(k) * kIn + f] = f < 9 * kTaps ? row[f] * inv : (mx > 0.0f ? row[f] : 0.0f);
+    S[k] = mx;
+    for (int ch = 0; ch < 9; ++ch) B[static_cast<long long>(k) * kOut + ch] = row[ch * kTaps + 3];   // tap 3 = the cell
+}
+
+// Face-frame scaled values at the cells of grid `tg` mapped into the grid of R by (ox, oy, oz).
+__global__ void face_values_kernel(Frame R, DevGrid tg, FrameCells C, int ox, int oy, int oz, float* out) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= C.count) return;
+    const int i = C.cell[k];
+    const unsigned char code = C.code[k];
+    const int axis = code >> 1;
+    const bool hi = (code & 1) != 0;
+    const int x = coord_of(tg, i, 0), y = coord_of(tg, i, 1), z = coord_of(tg, i, 2);
+    const long long p = (static_cast<long long>(z + oz) * R.g.gy + (y + oy)) * R.g.px + (x + ox);
+    const float zimp = impedance(R, p);
+    for (int ch = 0; ch < 9; ++ch) {
+        float v = R.f[kFaceField[axis][ch]][p];
+        if (!hi && kOdd[ch] != 0.0f) v = -v;
+        if (ch >= 3) v = v / zimp;
+        out[static_cast<long long>(k) * kOut + ch] = v;
+    }
+}
+
+// Rows idx[k] of a [*][width] array into dst rows first + k.
+__global__ void gather_rows_kernel(const float* src, const int* idx, int n, int width, float* dst, long long first) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(n) * width) return;
+    const long long k = t / width;
+    const int c = int(t % width);
+    dst[(first + k) * width + c] = src[static_cast<long long>(idx[k]) * width + c];
+}
+
+__global__ void targets_kernel(const float* V, const float* B, const float* S, int n, float* T) {
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<long long>(n) * kOut) return;
+    const float s = S[k / kOut];
+    T[k] = s > 0.0f ? (V[k] - B[k]) / s : 0.0f;
+}
+
+struct BoxArgs {
+    DevGrid dst, src;
+    float* df[9];
+    const float* sf[9];
+    int ox, oy, oz;
+};
+__global__ void copy_box_kernel(BoxArgs A) {
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x), y = int(blockIdx.y), z = int(blockIdx.z);
+    if (x >= A.dst.gx) return;
+    const long long d = (static_cast<long long>(z) * A.dst.gy + y) * A.dst.px + x;
+    const bool ghost = x < 4 || x >= A.dst.gx - 4 || y < 4 || y >= A.dst.gy - 4 || z < 4 || z >= A.dst.gz - 4;
+    const long long s = (static_cast<long long>(z + A.oz) * A.src.gy + (y + A.oy)) * A.src.px + (x + A.ox);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A.df[c][d] = ghost ? 0.0f : A.sf[c][s];
+}
+
+// ---- training --------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void forward_kernel(const void* image, const float* X, int n, float* H1, float* H2,
+                                                         float* Y) {
+    extern __shared__ unsigned char smem[];
+    Lds& L = *reinterpret_cast<Lds*>(smem);
+    load_image(L, image);
+    __syncthreads();
+    const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const int tiles = (n + 15) / 16;
+    const int waves_total = int(gridDim.x) * kWaves;
+    const int iters = (tiles + waves_total - 1) / waves_total;
+    for (int it = 0; it < iters; ++it) {
+        const int tile = it * waves_total + int(blockIdx.x) * kWaves + wave;
+        const long long first = static_cast<long long>(tile) * 16;
+        const int rows = tile < tiles ? int(min(16LL, n - first)) : 0;
+        const int cellr = lane >> 2, part = lane & 3;
+        for (int f = part; f < kIn; f += 4)
+            L.xs[wave][cellr][f] = to_bf16(cellr < rows ? X[(first + cellr) * kIn + f] : 0.0f);
+        __syncthreads();
+        const floatx4 out = mlp_tile(L, wave, H1, H2, first, rows);
+        const int m = lane & 15, q = lane >> 4;
+        if (m < kOut)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (4 * q + j < rows) Y[(first + 4 * q + j) * kOut + m] = out[j];
+        __syncthreads();
+    }
+}
+
+// One thread per sample: dY, dH2, dH1 through the BF16-rounded weights; squared error.
+__global__ void backward_samples_kernel(const void* image, const float* H1, const float* H2, const float* Y,
+                                        const float* T, int n, float denom, float* dY, float* dH2, float* dH1,
+                                        float* err) {
+    __shared__ float w2[kHidden * kHidden], w3[kOutPad * kHidden];
+    const unsigned short* w = static_cast<const unsigned short*>(image);
+    for (int k = threadIdx.x; k < kHidden * kHidden; k += blockDim.x) w2[k] = from_bf16(w[kHidden * kIn + k]);
+    for (int k = threadIdx.x; k < kOutPad * kHidden; k += blockDim.x)
+        w3[k] = from_bf16(w[kHidden * kIn + kHidden * kHidden + k]);
+    __syncthreads();
+    const long long s = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float dy[kOut], e = 0.0f;
+    for (int o = 0; o < kOut; ++o) {
+        const float r = Y[s * kOut + o] - T[s * kOut + o];
+        e += r * r;
+        dy[o] = 2.0f * r / denom;
+        dY[s * kOut + o] = dy[o];
+    }
+    err[s] = e;
+    float dh2[kHidden];
+    for (int k = 0; k < kHidden; ++k) {
+        float acc = 0.0f;
+        for (int o = 0; o < kOut; ++o) acc += dy[o] * w3[o * kHidden + k];   // W3^T [o][k]
+        dh2[k] = H2[s * kHidden + k] > 0.0f ? acc : 0.0f;
+        dH2[s * kHidden + k] = dh2[k];
+    }
+    for (int i = 0; i < kHidden; ++i) {
+        float acc = 0.0f;
+        for (int k = 0; k < kHidden; ++k) acc += dh2[k] * w2[k * kHidden + i];   // W2^T [k][i]
+        dH1[s * kHidden + i] = H1[s * kHidden + i] > 0.0f ? acc : 0.0f;
+    }
+}
+
+// Partial parameter gradients over a contiguous chunk of samples; parameter p per thread loop.
+__global__ void param_grads_kernel(const float* X, const float* H1, const float* H2, const float* dY, const float* dH2,
+                                   const float* dH1, int n, int chunks, float* partial) {
+    const int chunk = int(blockIdx.x);
+    const long long per = (static_cast<long long>(n) + chunks - 1) / chunks;
+    const long long s0 = chunk * per, s1 = min(static_cast<long long>(n), s0 + per);
+    for (int p = threadIdx.x; p < kParams; p += blockDim.x) {
+        float acc = 0.0f;
+        if (p < kB1) {
+            const int f = p / kHidden, i = p % kHidden;
+            for (long long s = s0; s < s1; ++s) acc += from_bf16(to_bf16(X[s * kIn + f])) * dH1[s * kHidden + i];
+        } else if (p < kW2) {
+            const int i = p - kB1;
+            for (long long s = s0; s < s1; ++s) acc += dH1[s * kHidden + i];
+        } else if (p < kB2) {
+            const int i = (p - kW2) / kHidden, k = (p - kW2) % kHidden;
+            for (long long s = s0; s < s1; ++s) acc += H1[s * kHidden + i] * dH2[s * kHidden + k];
+        } else if (p < kW3) {
+            const int k = p - kB2;
+            for (long long s = s0; s < s1; ++s) acc += dH2[s * kHidden + k];
+        } else if (p < kB3) {
+            const int k = (p - kW3) / kOut, o = (p - kW3) % kOut;
+            for (long long s = s0; s < s1; ++s) acc += H2[s * kHidden + k] * dY[s * kOut + o];
+        } else {
+            const int o = p - kB3;
+            for (long long s = s0; s < s1; ++s) acc += dY[s * kOut + o];
+        }
+        partial[static_cast<long long>(chunk) * kParams + p] = acc;
+    }
+}
+
+__global__ void reduce_partials_kernel(const float* partial, int chunks, float* grad) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= kParams) return;
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += partial[static_cast<long long>(c) * kParams + p];
+    grad[p] = acc;
+}
+
+__global__ void pack_kernel(const float* p, unsigned short* image) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    float* bias = reinterpret_cast<float*>(image + kBf16Count);
+    if (k < kHidden * kIn) {   // W1^T [j][i] = W1[i][j]
+        const int j = k / kIn, i = k % kIn;
+        image[k] = to_bf16(p[kW1 + i * kHidden + j]);
+    } else if (k < kHidden * kIn + kHidden * kHidden) {
+        const int r = k - kHidden * kIn, j = r / kHidden, i = r % kHidden;
+        image[k] = to_bf16(p[kW2 + i * kHidden + j]);
+    } else if (k < kBf16Count) {
+        const int r = k - kHidden * kIn - kHidden * kHidden, o = r / kHidden, i = r % kHidden;
+        image[k] = o < kOut ? to_bf16(p[kW3 + i * kOut + o]) : static_cast<unsigned short>(0);
+    } else if (k < kBf16Count + kHidden) {
+        bias[k - kBf16Count] = p[kB1 + (k - kBf16Count)];
+    } else if (k < kBf16Count + 2 * kHidden) {
+        bias[k - kBf16Count] = p[kB2 + (k - kBf16Count - kHidden)];
+    } else if (k < kBf16Count + 2 * kHidden + kOutPad) {
+        const int o = k - kBf16Count - 2 * kHidden;
+        bias[k - kBf16Count] = o < kOut ? p[kB3 + o] : 0.0f;
+    }
+}
+
+__global__ void adam_kernel(float* p, float* m, float* v, const float* g, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= kParams) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+__global__ void sum_partial_kernel(const float* x, int n, float* partial) {
+    __shared__ float s[256];
+    float acc = 0.0f;
+    for (long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<long long>(gridDim.x) * blockDim.x)
+        acc += x[k];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ void sum_final_kernel(const float* partial, int n, float* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        float acc = 0.0f;
+        for (int k = 0; k < n; ++k) acc += partial[k];
+        *out = acc;
+    }
+}
+
+unsigned blocks_for(long long n, int per) { return unsigned(n > 0 ? (n + per - 1) / per : 1); }
+
+constexpr int kSumBlocks = 256;
+
+}  // namespace
+
+hipError_t launch_correct(const Frame& F, const FrameCells& cells, const void* image, float* delta, hipStream_t s) {
+    if (cells.count <= 0) return hipSuccess;
+    const int tiles = (cells.count + 15) / 16;
+    const unsigned blocks = unsigned(std::min(2048, (tiles + kWaves - 1) / kWaves));
+    hipLaunchKernelGGL(correct_kernel, dim3(blocks), dim3(kBlock), sizeof(Lds), s, F, cells, image, delta);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply(float* const f[9], const FrameCells& cells, const float* delta, hipStream_t s) {
+    if (cells.count <= 0) return hipSuccess;
+    FieldPtrs P;
+    for (int c = 0; c < 9; ++c) P.f[c] = f[c];
+    hipLaunchKernelGGL(apply_kernel_v, dim3(blocks_for(cells.count, 256)), dim3(256), 0, s, P, cells, delta);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const Frame& F, const FrameCells& cells, float* X, float* S, float* B, hipStream_t s) {
+    if (cells.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(features_kernel, dim3(blocks_for(cells.count, 128)), dim3(128), 0, s, F, cells, X, S, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_values(const Frame& R, const DevGrid& tgrid, const FrameCells& cells, int ox, int oy, int oz,
+                              float* out, hipStream_t s) {
+    if (cells.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(face_values_kernel, dim3(blocks_for(cells.count, 256)), dim3(256), 0, s, R, tgrid, cells, ox, oy,
+                       oz, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const float* src, const int* idx, int n, int width, float* dst, long long first,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for(static_cast<long long>(n) * width, 256)), dim3(256), 0, s,
+                       src, idx, n, width, dst, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_targets(const float* V, const float* B, const float* S, int n, float* T, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(targets_kernel, dim3(blocks_for(static_cast<long long>(n) * kOut, 256)), dim3(256), 0, s, V, B,
+                       S, n, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_box(const DevGrid& dst, float* const df[9], const DevGrid& src, const float* const sf[9],
+                           int ox, int oy, int oz, hipStream_t s) {
+    BoxArgs A{};
+    A.dst = dst;
+    A.src = src;
+    for (int c = 0; c < 9; ++c) { A.df[c] = df[c]; A.sf[c] = sf[c]; }
+    A.ox = ox; A.oy = oy; A.oz = oz;
+    hipLaunchKernelGGL(copy_box_kernel, dim3(blocks_for(dst.gx, 64), unsigned(dst.gy), unsigned(dst.gz)), dim3(64), 0, s,
+                       A);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const void* image, const float* X, int n, float* H1, float* H2, float* Y, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int tiles = (n + 15) / 16;
+    const unsigned blocks = unsigned(std::min(2048, (tiles + kWaves - 1) / kWaves));
+    hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(kBlock), sizeof(Lds), s, image, X, n, H1, H2, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward_samples(const void* image, const float* H1, const float* H2, const float* Y,
+                                       const float* T, int n, float denom, float* dY, float* dH2, float* dH1,
+                                       float* err, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(backward_samples_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, s, image, H1, H2, Y, T, n,
+                       denom, dY, dH2, dH1, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_param_grads(const float* X, const float* H1, const float* H2, const float* dY, const float* dH2,
+                                  const float* dH1, int n, float* partial, int chunks, float* grad, hipStream_t s) {
+    hipLaunchKernelGGL(param_grads_kernel, dim3(unsigned(chunks)), dim3(256), 0, s, X, H1, H2, dY, dH2, dH1, n, chunks,
+                       partial);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, partial, chunks, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, float* m, float* v, const float* grad, float lr, float beta1, float beta2, float eps,
+                       int step, void* image, hipStream_t s) {
+    const float c1 = 1.0f - std::pow(beta1, float(step)), c2 = 1.0f - std::pow(beta2, float(step));
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, p, m, v, grad, lr, beta1, beta2,
+                       eps, c1, c2);
+    return launch_pack(p, image, s);
+}
+
+hipError_t launch_pack(const float* p, void* image, hipStream_t s) {
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(kBf16Count + 2 * kHidden + kOutPad, 256)), dim3(256), 0, s, p,
+                       static_cast<unsigned short*>(image));
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(const float* x, int n, float* partial, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_partial_kernel, dim3(kSumBlocks), dim3(256), 0, s, x, n, partial);
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(64), 0, s, partial, kSumBlocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace seis3d::abc

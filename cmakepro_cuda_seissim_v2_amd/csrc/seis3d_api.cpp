@@ -235,6 +235,10 @@ void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent
     if (ev) check(hipEventRecord(ev[2], t->stream), "record event");
     check(seis3d::launch_stress(S, t->kernels, t->stream), "kernel launch");
     if (ev) check(hipEventRecord(ev[3], t->stream), "record event");
+    if (t->abc) {
+        enqueue_abc(t);
+        if (c.free_surface) check(seis3d::launch_velocity_ghost(P, c.spacing, t->stream), "kernel launch");
+    }
     if (c.free_surface) check(seis3d::launch_stress_ghost(P, t->stream), "kernel launch");
     if (c.reflecting) check(seis3d::launch_reflecting_walls(P, false, t->stream), "kernel launch");
     t->ghosts_valid = true;
@@ -510,6 +514,7 @@ int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx) {
             c.free_surface != t->c.free_surface || c.pml != t->c.pml || c.dt != t->c.dt)
             invalid("Seis3D: a CPML refresh may only change the source frequency.");
         if (c.reflecting) { t->c = c; return; }
+        if (t->abc) fail(SEIS3D_ELIFECYCLE, "Seis3D: detach the learned boundary before refreshing the CPML.");
         float* replacement[3]{};
         try {
             for (int a = 0; a < 3; ++a) {

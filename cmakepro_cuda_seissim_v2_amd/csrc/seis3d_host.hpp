@@ -67,6 +67,10 @@ struct Plan {
     std::size_t bytes = 0;
 };
 
+// validateContext3D: checks the context and derives the padded shape and dt.
+void validate(seis3d_ctx& c);
+// One material cell: finite, 0 < Vp <= 30000, 0 <= Vs < Vp, positive bulk modulus, 0 < rho <= 50000.
+bool valid_material(float vp, float vs, float rho);
 Plan plan_for(const seis3d_ctx& c);
 std::vector<float> make_coefficients(const seis3d_ctx& c, int axis);
 float ricker(float frequency, double time);
@@ -76,6 +80,11 @@ float ricker(float frequency, double time);
 // ---------------------------------------------------------------------------
 // Session
 // ---------------------------------------------------------------------------
+
+namespace seis3d::host {
+struct AbcFrame;                                   // learned-boundary state (seis3d_abc.cpp)
+void release_abc(AbcFrame*& a) noexcept;
+}  // namespace seis3d::host
 
 struct seis3d_session {
     using Plan = seis3d::host::Plan;
@@ -101,6 +110,7 @@ struct seis3d_session {
     bool recording = false;
     bool ghosts_valid = true;          // σ ghost planes consistent with the interior
     seis3d::KernelConfig kernels;
+    seis3d::host::AbcFrame* abc = nullptr;   // learned absorbing boundary, when attached
 
     seis3d::DevStep base() const {
         seis3d::DevStep P{};
@@ -134,6 +144,7 @@ struct seis3d_session {
 
     void release() noexcept {
         if (stream) (void)hipStreamSynchronize(stream);
+        seis3d::host::release_abc(abc);
         for (auto& p : fields) { (void)hipFree(p); p = nullptr; }
         (void)hipFree(inv_rho); (void)hipFree(lambda); (void)hipFree(mu); (void)hipFree(inv_mu);
         inv_rho = lambda = mu = inv_mu = nullptr;
@@ -157,5 +168,7 @@ seis3d::DevSource make_source(const seis3d_ctx& c, const seis3d_source& s, doubl
 // One runStep3D (cuda/Cuda_Seis3D.cu:602-624), enqueued on the session stream;
 // optional events bracket the two fused half-steps.
 void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev = nullptr);
+// Learned-boundary correction of the frame cells after the stress half-step (no-op when detached).
+void enqueue_abc(seis3d_t* t);
 
 }  // namespace seis3d::host

@@ -4,7 +4,9 @@ Names follow the reference's operator interface (cuda/Cuda_Seis3D.cuh:6-42):
 ``validateContext3D``, ``estimateMemory3D``, ``initializeGpu3D``, ``releaseGpu3D``,
 ``resetGpu3D``, ``refreshCpmlCoefficients3D``, ``runStep3D``, ``setReceivers3D``,
 ``sampleReceivers3D``, ``probeMaterial3D``, ``uploadModelPlane3D``,
-``finishModelUpload3D``, ``downloadModelPlane3D``; error behaviour maps the C codes
+``finishModelUpload3D``, ``downloadModelPlane3D``, and the shot-record and model files
+``exportGather3D``, ``readShotSegy3D``, ``inspectModel3D``, ``importModel3D``,
+``exportModel3D`` (src/Seis3D/Acquisition3D.cpp, ShotRecordIo3D.cpp, ModelIo3D.cpp); error behaviour maps the C codes
 back to the reference's exception classes (invalid_argument -> ``InvalidArgument``
 (a ValueError), runtime_error -> ``SolverRuntimeError``, logic_error -> ``LogicError``).
 
@@ -13,6 +15,7 @@ All compute runs in the HIP library ``lib/libseis3d.so``; there is no CPU fallba
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -101,6 +104,21 @@ class Shot(C.Structure):
                 ("observed", C.POINTER(C.c_float))]
 
 
+class GatherInfo(C.Structure):
+    """seis3d_gather_info: frozen acquisition metadata of one shot gather (Gather3D)."""
+    _fields_ = [("spacing", C.c_float), ("frequency", C.c_float), ("source", Source),
+                ("world_origin", C.c_double * 2), ("cmp_bin", C.c_float * 2), ("shot_id", C.c_int32),
+                ("stride", C.c_int32), ("samples", C.c_int32), ("sample_interval_us", C.c_int32),
+                ("coordinate_scalar", C.c_int32), ("first_sample_time", C.c_double)]
+
+
+class SegyInfo(C.Structure):
+    """seis3d_segy_info: header scan of one single-component shot file."""
+    _fields_ = [("traces", C.c_int32), ("samples", C.c_int32), ("sample_interval_us", C.c_int32),
+                ("format", C.c_int32), ("revision", C.c_int32), ("component", C.c_int32),
+                ("shot_id", C.c_int32), ("measurement_unit", C.c_int32), ("coordinate_units", C.c_int32)]
+
+
 class Timing(C.Structure):
     _fields_ = [("velocity_ms", C.c_double), ("stress_ms", C.c_double), ("total_ms", C.c_double),
                 ("launches", C.c_int64)]
@@ -168,6 +186,31 @@ def library() -> C.CDLL:
             "seis3d_batch_launch": (C.c_int, [vp]),
             "seis3d_batch_synchronize": (C.c_int, [vp]),
             "seis3d_batch_destroy": (None, [vp]),
+            "seis3d_abc_param_count": (C.c_int, []),
+            "seis3d_abc_attach": (C.c_int, [vp, P(C.c_float)]),
+            "seis3d_abc_detach": (C.c_int, [vp]),
+            "seis3d_abc_frame_cells": (C.c_int, [vp, P(C.c_int)]),
+            "seis3d_copy_wavefield_box": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
+            "seis3d_abc_trainer_create": (C.c_int, [C.c_int, C.c_uint64, P(vp)]),
+            "seis3d_abc_trainer_destroy": (None, [vp]),
+            "seis3d_abc_trainer_count": (C.c_int, [vp]),
+            "seis3d_abc_trainer_add": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                                 P(C.c_int)]),
+            "seis3d_abc_train": (C.c_int, [vp, P(C.c_float), C.c_int, C.c_int, C.c_float, P(C.c_float)]),
+            "seis3d_abc_evaluate": (C.c_int, [vp, P(C.c_float), P(C.c_double)]),
+            "seis3d_abc_mlp": (C.c_int, [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int, P(C.c_float),
+                                         P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_double)]),
+            "seis3d_gather_info_init": (C.c_int, [P(Ctx), P(Source), P(Receiver), C.c_int, C.c_int, C.c_int,
+                                                  C.c_int, P(C.c_double), P(C.c_float), P(GatherInfo)]),
+            "seis3d_export_gather": (C.c_int, [P(GatherInfo), P(Receiver), C.c_int, P(C.c_float), C.c_char_p,
+                                               C.c_char_p]),
+            "seis3d_segy_inspect": (C.c_int, [C.c_char_p, P(SegyInfo)]),
+            "seis3d_segy_read": (C.c_int, [C.c_char_p, P(C.c_float), P(C.c_double), P(C.c_double),
+                                           P(C.c_int32)]),
+            "seis3d_s3dm_inspect": (C.c_int, [C.c_char_p, P(Ctx)]),
+            "seis3d_s3dm_load": (C.c_int, [vp, C.c_char_p]),
+            "seis3d_s3dm_save": (C.c_int, [vp, C.c_char_p]),
+            "seis3d_s3dm_write": (C.c_int, [C.c_char_p, P(Ctx), P(C.c_float)]),
             "seis3d_fwi_plan": (C.c_int, [P(Ctx), C.c_int64, C.c_int, C.c_int, C.c_int, P(C.c_uint64),
                                           P(C.c_int)]),
             "seis3d_fwi_gradient": (C.c_int, [vp, P(Shot), C.c_int, C.c_int64, C.c_int, C.c_int, P(C.c_double),
@@ -485,6 +528,38 @@ class GPUSimData3D:
         _check(library().seis3d_get_tiles(self._h, C.byref(f), C.byref(y), C.byref(z)))
         return f.value, y.value, z.value
 
+    # ---- learned absorbing boundary (no reference counterpart) ----
+
+    def abc_attach(self, params=None) -> None:
+        """Learned 8-cell frame (pml must be 8); params None gives the plain update (training base)."""
+        if params is None:
+            _check(library().seis3d_abc_attach(self._h, None))
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            if p.size != library().seis3d_abc_param_count():
+                raise InvalidArgument("wrong parameter count")
+            _check(library().seis3d_abc_attach(self._h, _fptr(p)))
+
+    def abc_detach(self) -> None:
+        _check(library().seis3d_abc_detach(self._h))
+
+    def abc_frame_cells(self) -> int:
+        n = C.c_int()
+        _check(library().seis3d_abc_frame_cells(self._h, C.byref(n)))
+        return n.value
+
+    def copy_wavefield_box(self, src: "GPUSimData3D", ox: int, oy: int, oz: int) -> None:
+        """This session's non-ghost wavefield from src's padded grid at offset (ox, oy, oz)."""
+        _check(library().seis3d_copy_wavefield_box(self._h, src._h, int(ox), int(oy), int(oz)))
+
+    def load_model(self, path) -> None:
+        """Stream a .s3dm file of this session's shape into the device model (importModel3D)."""
+        _check(library().seis3d_s3dm_load(self._h, os.fsencode(path)))
+
+    def save_model(self, path) -> None:
+        """Write the device model to a new .s3dm file (exportModel3D)."""
+        _check(library().seis3d_s3dm_save(self._h, os.fsencode(path)))
+
     def fwi_gradient(self, shots, n_steps: int, stride: int, checkpoint_interval: int = 0):
         """Misfit and gradient ([nz, ny, nx] float32 vp, vs, rho) of shots (source, receivers, observed).
 
@@ -552,6 +627,83 @@ def fwi_plan(ctx: Ctx, n_steps: int, stride: int, checkpoint_interval: int, rece
     return {"shot_bytes": b.value, "interval": k.value}
 
 
+def _receiver_array(receivers):
+    rec = [tuple(int(v) for v in r) for r in receivers]
+    arr = (Receiver * max(1, len(rec)))()
+    for i, r in enumerate(rec):
+        arr[i] = Receiver(*r)
+    return arr, len(rec)
+
+
+def gather_info(ctx: Ctx, src: Source, receivers, shot_id: int, stride: int, samples: int,
+                world_origin=(0.0, 0.0), cmp_bin=(10.0, 10.0)) -> GatherInfo:
+    """Frozen gather metadata (startGather3D): sample interval, first sample time, XY scalar."""
+    arr, n = _receiver_array(receivers)
+    origin = (C.c_double * 2)(*world_origin)
+    bins = (C.c_float * 2)(*cmp_bin)
+    out = GatherInfo()
+    _check(library().seis3d_gather_info_init(C.byref(ctx), C.byref(src), arr, n, int(shot_id), int(stride),
+                                             int(samples), origin, bins, C.byref(out)))
+    return out
+
+
+def export_gather(info: GatherInfo, receivers, samples: np.ndarray, directory, prefix: str = "shot") -> list:
+    """Write <prefix>_<shot:04>_v{x,y,z}.sgy and _geometry.csv (exportGather3D); returns the paths.
+
+    samples is the gather [sample][receiver][vx, vy, vz]."""
+    arr, n = _receiver_array(receivers)
+    a = np.ascontiguousarray(samples, dtype=np.float32)
+    if a.shape != (info.samples, n, 3):
+        raise InvalidArgument(f"gather shape {a.shape} != ({info.samples}, {n}, 3)")
+    _check(library().seis3d_export_gather(C.byref(info), arr, n, _fptr(a), os.fsencode(directory),
+                                          prefix.encode()))
+    base = Path(directory) / f"{prefix}_{info.shot_id:04d}"
+    return [Path(f"{base}{e}") for e in ("_vx.sgy", "_vy.sgy", "_vz.sgy", "_geometry.csv")]
+
+
+def inspect_segy(path) -> SegyInfo:
+    info = SegyInfo()
+    _check(library().seis3d_segy_inspect(os.fsencode(path), C.byref(info)))
+    return info
+
+
+def read_shot_segy(path) -> dict:
+    """One single-component shot (readShotSegy3D): samples [trace][sample], receiver coordinates
+    [trace][x, y, z-depth] in metres, first-sample times, trace ids, and integer receiver keys that
+    are the coordinate ranks along each axis (world metres are never rounded onto grid nodes)."""
+    info = inspect_segy(path)
+    samples = np.zeros((info.traces, info.samples), np.float32)
+    coords = np.zeros((info.traces, 3), np.float64)
+    times = np.zeros(info.traces, np.float64)
+    ids = np.zeros(info.traces, np.int32)
+    _check(library().seis3d_segy_read(os.fsencode(path), _fptr(samples),
+                                      coords.ctypes.data_as(C.POINTER(C.c_double)),
+                                      times.ctypes.data_as(C.POINTER(C.c_double)),
+                                      ids.ctypes.data_as(C.POINTER(C.c_int32))))
+    ranks = np.stack([np.searchsorted(np.unique(coords[:, a]), coords[:, a]) for a in range(3)], axis=1)
+    units = ["file units", "Pa", "V", "mV", "A", "m", "m/s", "m/s^2", "N", "W"]
+    m = info.measurement_unit
+    return {"info": info, "samples": samples, "coordinates": coords, "first_times": times, "trace_ids": ids,
+            "receivers": ranks.astype(np.int32), "first_sample_time": float(times[0]),
+            "component": ("Vx", "Vy", "Vz")[info.component],
+            "amplitude_unit": units[m] if 1 <= m <= 9 else units[0]}
+
+
+def inspect_model(path, ctx: Ctx | None = None) -> Ctx:
+    """Header, payload size and every material cell of a .s3dm file (inspectModel3D)."""
+    c = (ctx or default_context()).copy()
+    _check(library().seis3d_s3dm_inspect(os.fsencode(path), C.byref(c)))
+    return c
+
+
+def write_model(path, ctx: Ctx, vp: np.ndarray, vs: np.ndarray, rho: np.ndarray) -> None:
+    """Write host planes [nz][ny][nx] of Vp, Vs, rho as a new .s3dm file."""
+    planes = np.ascontiguousarray(np.stack([vp, vs, rho], axis=-1), dtype=np.float32)
+    if planes.shape != (ctx.nz, ctx.ny, ctx.nx, 3):
+        raise InvalidArgument(f"model shape {planes.shape[:3]} != ({ctx.nz}, {ctx.ny}, {ctx.nx})")
+    _check(library().seis3d_s3dm_write(os.fsencode(path), C.byref(ctx), _fptr(planes)))
+
+
 def device_count() -> int:
     return int(library().seis3d_device_count())
 
@@ -599,3 +751,23 @@ def finishModelUpload3D(g: GPUSimData3D) -> None:
 
 def downloadModelPlane3D(g: GPUSimData3D, z: int) -> np.ndarray:
     return g.download_plane(z)
+
+
+def exportGather3D(info: GatherInfo, receivers, samples: np.ndarray, directory, prefix: str = "shot") -> list:
+    return export_gather(info, receivers, samples, directory, prefix)
+
+
+def readShotSegy3D(path) -> dict:
+    return read_shot_segy(path)
+
+
+def inspectModel3D(path, ctx: Ctx | None = None) -> Ctx:
+    return inspect_model(path, ctx)
+
+
+def importModel3D(g: GPUSimData3D, path) -> None:
+    g.load_model(path)
+
+
+def exportModel3D(g: GPUSimData3D, path) -> None:
+    g.save_model(path)

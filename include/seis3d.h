@@ -278,6 +278,101 @@ int seis3d_fwi_plan(const seis3d_ctx* ctx, int64_t n_steps, int stride, int chec
 int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_t n_steps, int stride,
                         int checkpoint_interval, double* misfit, float* g_vp, float* g_vs, float* g_rho);
 
+/* ---- Shot records and model files (host only) ------------------------------ */
+
+/* Frozen acquisition metadata of one gather (Gather3D, include/Seis3D_Sim.h:47-57,
+ * with Survey3D's world origin and CMP bin size, :36-45). */
+typedef struct seis3d_gather_info {
+    float spacing, frequency;          /* context the gather was recorded with */
+    seis3d_source source;
+    double world_origin[2];
+    float cmp_bin[2];
+    int32_t shot_id;
+    int32_t stride;
+    int32_t samples;
+    int32_t sample_interval_us;
+    int32_t coordinate_scalar;         /* SEG-Y XY scalar (-1000, -100, -10 or 1) */
+    double first_sample_time;
+} seis3d_gather_info;
+
+/* Acquisition metadata of startGather3D (src/Seis3D/Acquisition3D.cpp:188-230):
+ * validates source, sample count (1..32767), stride (1..10000), CMP bins and
+ * world origin; interval = llround(dt 1e6) stride (<= 32767 us); first sample
+ * time (stride - 1/2) dt; the largest XY scalar that keeps every coordinate
+ * in 32-bit range. */
+int seis3d_gather_info_init(const seis3d_ctx* ctx, const seis3d_source* src, const seis3d_receiver* receivers,
+                            int count, int shot_id, int stride, int samples, const double world_origin[2],
+                            const float cmp_bin[2], seis3d_gather_info* out);
+/* exportGather3D (src/Seis3D/Acquisition3D.cpp:157-185): <prefix>_<shot:04>_vx.sgy,
+ * _vy.sgy, _vz.sgy (SEG-Y rev1, IEEE float, big-endian; writeSegyComponent3D,
+ * :76-154) and _geometry.csv in `directory`; refuses to overwrite any of them.
+ * samples is the gather [sample][receiver][vx,vy,vz]. */
+int seis3d_export_gather(const seis3d_gather_info* info, const seis3d_receiver* receivers, int count,
+                         const float* samples, const char* directory, const char* prefix);
+
+/* One imported single-component shot (readShotSegy3D, src/Seis3D/ShotRecordIo3D.cpp:66-141). */
+typedef struct seis3d_segy_info {
+    int32_t traces, samples, sample_interval_us, format, revision;
+    int32_t component;                 /* 0 vx (id 14), 1 vy (13), 2 vz (12 or other) */
+    int32_t shot_id;
+    int32_t measurement_unit;          /* trace bytes 203-204 (6: m/s) */
+    int32_t coordinate_units;          /* trace bytes 89-90 (1: length) */
+} seis3d_segy_info;
+/* Header scan and full validation of a rev0/rev1 big-endian file (formats 1, 2, 3, 5, 8). */
+int seis3d_segy_inspect(const char* path, seis3d_segy_info* info);
+/* Samples [trace][sample], receiver coordinates [trace][x, y, z] (metres, z as depth, positive down),
+ * first-sample times [trace] and trace ids [trace]; any output may be null. */
+int seis3d_segy_read(const char* path, float* samples, double* coordinates, double* first_times, int32_t* trace_ids);
+
+/* inspectModel3D (src/Seis3D/ModelIo3D.cpp:35-65): header, payload size and every
+ * material cell of a .s3dm file; fills nx, ny, nz, pml, boundaries, spacing,
+ * max_vp and validates.  Other members of ctx are kept. */
+int seis3d_s3dm_inspect(const char* path, seis3d_ctx* ctx);
+/* importModel3D (:68-77): stream the planes into a session of the same shape. */
+int seis3d_s3dm_load(seis3d_t* t, const char* path);
+/* exportModel3D (:80-102): a new file only, written through <path>.part. */
+int seis3d_s3dm_save(seis3d_t* t, const char* path);
+/* The same file from host planes [z][y][x][vp, vs, rho] (no device needed). */
+int seis3d_s3dm_write(const char* path, const seis3d_ctx* ctx, const float* vp_vs_rho);
+
+/* ---- Learned absorbing boundary (no reference counterpart) ---------------- */
+
+/* Number of FP32 parameters of the frame MLP: W1[96][32], b1[32], W2[32][32],
+ * b2[32], W3[32][9], b3[9] (row-major, input index first). */
+int seis3d_abc_param_count(void);
+/* Replace the 8 CPML cells of every absorbing face (pml must be 8; a free top
+ * keeps its free surface) by the learned frame: the frame takes the plain
+ * update and, when params is not null, the MLP's correction after every
+ * step (BF16 weights on MFMA).  params null leaves the plain update only
+ * (the training base).  Calls again replace the parameters. */
+int seis3d_abc_attach(seis3d_t* t, const float* params);
+/* Restore the session's CPML: plain 8-cell CPML is the fallback. */
+int seis3d_abc_detach(seis3d_t* t);
+int seis3d_abc_frame_cells(const seis3d_t* t, int* count);
+/* Copy the 9 wavefields of dst's non-ghost cells from src at cell offset
+ * (ox, oy, oz) of src's padded grid; dst's ghost cells are cleared. */
+int seis3d_copy_wavefield_box(seis3d_t* dst, const seis3d_t* src, int ox, int oy, int oz);
+
+/* GPU trainer of the frame MLP.  Samples are teacher-forced: `thin` (learned
+ * frame attached without parameters) holds one plain step taken from the
+ * state of `wide` before `wide` advanced one step; the targets are wide's
+ * values at the frame cells (thin cell + (ox, oy, oz)) minus thin's, in the
+ * MLP's normalised face frame.  Up to max_cells random frame cells whose
+ * field scale exceeds min_rel_scale times the largest are added. */
+typedef struct seis3d_abc_trainer seis3d_abc_trainer_t;
+int seis3d_abc_trainer_create(int capacity, uint64_t seed, seis3d_abc_trainer_t** out);
+void seis3d_abc_trainer_destroy(seis3d_abc_trainer_t* tr);
+int seis3d_abc_trainer_count(const seis3d_abc_trainer_t* tr);
+int seis3d_abc_trainer_add(seis3d_abc_trainer_t* tr, seis3d_t* thin, const seis3d_t* wide, int ox, int oy, int oz,
+                           int max_cells, float min_rel_scale, int* added);
+/* Adam over shuffled mini-batches (cosine-decayed rate); params in/out; mean squared error per epoch. */
+int seis3d_abc_train(seis3d_abc_trainer_t* tr, float* params, int epochs, int batch, float lr, float* epoch_loss);
+int seis3d_abc_evaluate(seis3d_abc_trainer_t* tr, const float* params, double* loss);
+/* The training kernels on host data: forward (Y [n][9], BF16-rounded H1/H2
+ * [n][32]), mean squared error against T and the parameter gradient. */
+int seis3d_abc_mlp(const float* params, const float* X, const float* T, int n, float* Y, float* H1, float* H2,
+                   float* grad, double* loss);
+
 /* ---- Errors and build info ------------------------------------------------ */
 
 const char* seis3d_last_error(void);
