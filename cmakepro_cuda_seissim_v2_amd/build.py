@@ -74,7 +74,8 @@ def build_oracle() -> Path:
     return out
 
 
-CPP_TESTS = {"seis3d_boundary_regression": "seis3d", "seis2d_solver_regression": "seis2d"}
+CPP_TESTS = {"seis3d_boundary_regression": "seis3d", "seis2d_solver_regression": "seis2d",
+             "seis3d_io_regression": "seis3d"}
 
 
 def build_cpp_tests() -> list[Path]:

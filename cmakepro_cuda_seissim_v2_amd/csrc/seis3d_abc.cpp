@@ -147,6 +147,8 @@ struct seis3d_abc_trainer {
 namespace {
 
 constexpr int kChunks = 128;
+constexpr float kMaxTarget = 4.0f;   // largest |normalised target| a training row may carry
+constexpr int kMaxRetries = 6;       // diverged epochs redone at half rate before giving up
 
 struct Scratch {
     std::vector<void*> ptrs;
@@ -298,18 +300,24 @@ int seis3d_abc_trainer_add(seis3d_abc_trainer_t* tr, seis3d_t* thin, const seis3
         float* Ball = sc.get<float>(std::size_t(L) * kOut);
         float* Vall = sc.get<float>(std::size_t(L) * kOut);
         float* Tall = sc.get<float>(std::size_t(L) * kOut);
+        float* Mall = sc.get<float>(std::size_t(L));
         const Frame Ft = frame_of(thin), Fw = frame_of(wide);
         const FrameCells C = cells_of(thin->abc);
         check(launch_features(Ft, C, Xall, Sall, Ball, tr->stream), "kernel launch");
         check(launch_face_values(Fw, Ft.g, C, ox, oy, oz, Vall, tr->stream), "kernel launch");
-        check(launch_targets(Vall, Ball, Sall, L, Tall, tr->stream), "kernel launch");
-        std::vector<float> hs(static_cast<std::size_t>(L));
+        check(launch_targets(Vall, Ball, Sall, L, Tall, Mall, tr->stream), "kernel launch");
+        std::vector<float> hs(static_cast<std::size_t>(L)), hm(static_cast<std::size_t>(L));
         check(hipMemcpyAsync(hs.data(), Sall, hs.size() * sizeof(float), hipMemcpyDeviceToHost, tr->stream), "read scales");
+        check(hipMemcpyAsync(hm.data(), Mall, hm.size() * sizeof(float), hipMemcpyDeviceToHost, tr->stream), "read scales");
         check(hipStreamSynchronize(tr->stream), "read scales");
         const float top = hs.empty() ? 0.0f : *std::max_element(hs.begin(), hs.end());
+        // A one-step correction is a fraction of its window's largest value; rows
+        // whose scale does not bound the target (a front entering an almost silent
+        // window) are outliers that would dominate the squared loss.
         std::vector<int> cand;
         for (int k = 0; k < L; ++k)
-            if (top > 0.0f && hs[std::size_t(k)] > min_rel_scale * top) cand.push_back(k);
+            if (top > 0.0f && hs[std::size_t(k)] > min_rel_scale * top && hm[std::size_t(k)] <= kMaxTarget)
+                cand.push_back(k);
         const int room = tr->capacity - tr->count;
         const int take = std::max(0, std::min({max_cells, room, int(cand.size())}));
         for (int k = 0; k < take; ++k) {   // partial Fisher-Yates
@@ -364,9 +372,20 @@ int seis3d_abc_train(seis3d_abc_trainer_t* tr, float* params, int epochs, int ba
         check(launch_pack(tr->p, tr->image, s), "kernel launch");
         int step = 0;
         std::vector<float> losses(static_cast<std::size_t>(batches));
+        std::vector<float> host_p(static_cast<std::size_t>(kParams));
+        float* backup = sc.get<float>(3 * std::size_t(kParams));
+        float damping = 1.0f;
+        int retries = 0;
         for (int e = 0; e < epochs; ++e) {
-            // Cosine decay from lr to lr / 20.
-            const float rate = lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * float(e) / float(epochs))));
+            // Cosine decay from lr to lr / 20 (halved after each diverged epoch).
+            const float rate = damping * lr *
+                               (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * float(e) / float(epochs))));
+            const int step0 = step;
+            check(hipMemcpyAsync(backup, tr->p, kParams * sizeof(float), hipMemcpyDeviceToDevice, s), "save state");
+            check(hipMemcpyAsync(backup + kParams, tr->m, kParams * sizeof(float), hipMemcpyDeviceToDevice, s),
+                  "save state");
+            check(hipMemcpyAsync(backup + 2 * kParams, tr->v, kParams * sizeof(float), hipMemcpyDeviceToDevice, s),
+                  "save state");
             for (int b = 0; b < batches; ++b) {
                 const int first = b * batch, rows = std::min(batch, n - first);
                 forward_backward(s, tr->image, Xs + std::size_t(first) * kIn, Ts + std::size_t(first) * kOut, rows,
@@ -377,9 +396,28 @@ int seis3d_abc_train(seis3d_abc_trainer_t* tr, float* params, int epochs, int ba
             }
             check(hipMemcpyAsync(losses.data(), tr->loss, losses.size() * sizeof(float), hipMemcpyDeviceToHost, s),
                   "read losses");
+            check(hipMemcpyAsync(host_p.data(), tr->p, kParams * sizeof(float), hipMemcpyDeviceToHost, s),
+                  "read parameters");
             check(hipStreamSynchronize(s), "train");
             double total = 0;
             for (float l : losses) total += double(l);
+            bool finite = std::isfinite(total);
+            for (float v : host_p) finite = finite && std::isfinite(v);
+            if (!finite) {
+                // Diverged: restore the epoch's starting state and redo it at half the rate.
+                if (++retries > kMaxRetries)
+                    fail(SEIS3D_ENONFINITE, "Seis3D: learned-boundary training diverged; lower the learning rate.");
+                check(hipMemcpyAsync(tr->p, backup, kParams * sizeof(float), hipMemcpyDeviceToDevice, s), "restore state");
+                check(hipMemcpyAsync(tr->m, backup + kParams, kParams * sizeof(float), hipMemcpyDeviceToDevice, s),
+                      "restore state");
+                check(hipMemcpyAsync(tr->v, backup + 2 * kParams, kParams * sizeof(float), hipMemcpyDeviceToDevice, s),
+                      "restore state");
+                check(launch_pack(tr->p, tr->image, s), "kernel launch");
+                step = step0;
+                damping *= 0.5f;
+                --e;
+                continue;
+            }
             if (epoch_loss) epoch_loss[e] = float(total / (double(n) * kOut));
         }
         check(hipMemcpyAsync(params, tr->p, kParams * sizeof(float), hipMemcpyDeviceToHost, s), "read parameters");

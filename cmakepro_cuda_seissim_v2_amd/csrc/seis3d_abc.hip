@@ -267,11 +267,20 @@ __global__ void gather_rows_kernel(const float* src, const int* idx, int n, int 
     dst[(first + k) * width + c] = src[static_cast<long long>(idx[k]) * width + c];
 }
 
-__global__ void targets_kernel(const float* V, const float* B, const float* S, int n, float* T) {
-    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= static_cast<long long>(n) * kOut) return;
-    const float s = S[k / kOut];
-    T[k] = s > 0.0f ? (V[k] - B[k]) / s : 0.0f;
+// Normalised targets of each row and their largest magnitude (infinity when a
+// target is not finite), so the caller can drop rows the scale does not bound.
+__global__ void targets_kernel(const float* V, const float* B, const float* S, int n, float* T, float* M) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    const float s = S[k];
+    float mx = 0.0f;
+    for (int c = 0; c < kOut; ++c) {
+        const long long j = static_cast<long long>(k) * kOut + c;
+        const float t = s > 0.0f ? (V[j] - B[j]) / s : 0.0f;
+        T[j] = t;
+        mx = isfinite(t) ? fmaxf(mx, fabsf(t)) : INFINITY;
+    }
+    M[k] = mx;
 }
 
 struct BoxArgs {
@@ -494,10 +503,9 @@ hipError_t launch_gather_rows(const float* src, const int* idx, int n, int width
     return hipGetLastError();
 }
 
-hipError_t launch_targets(const float* V, const float* B, const float* S, int n, float* T, hipStream_t s) {
+hipError_t launch_targets(const float* V, const float* B, const float* S, int n, float* T, float* M, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(targets_kernel, dim3(blocks_for(static_cast<long long>(n) * kOut, 256)), dim3(256), 0, s, V, B,
-                       S, n, T);
+    hipLaunchKernelGGL(targets_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, V, B, S, n, T, M);
     return hipGetLastError();
 }
 

@@ -52,8 +52,9 @@ hipError_t launch_face_values(const Frame& R, const DevGrid& tgrid, const FrameC
 // dst rows first.. = src rows idx[0..n) of width floats.
 hipError_t launch_gather_rows(const float* src, const int* idx, int n, int width, float* dst, long long first,
                               hipStream_t s);
-// Normalised targets T = (V - B) / S (0 where S = 0).
-hipError_t launch_targets(const float* V, const float* B, const float* S, int n, float* T, hipStream_t s);
+// Normalised targets T = (V - B) / S (0 where S = 0) and M = max |T| per row
+// (infinity for a row with a non-finite target).
+hipError_t launch_targets(const float* V, const float* B, const float* S, int n, float* T, float* M, hipStream_t s);
 // Copy of the non-ghost cells of one grid from another at a cell offset; ghost cells cleared.
 hipError_t launch_copy_box(const DevGrid& dst, float* const df[9], const DevGrid& src, const float* const sf[9],
                            int ox, int oy, int oz, hipStream_t s);

@@ -53,11 +53,12 @@ __device__ __forceinline__ void copy_row(const float* src, float* lds_row) {
 }
 
 // Copy ROWS rows of one plane into an LDS tile of width W (72: x halo,
-// starting at column x0 - 4; 64: starting at x0).  Wave ty copies rows
-// ty, ty + BY, ...  A 72-wide row takes two full-wave copies, columns 0..63
-// and 8..71: the overlap is written twice with identical values, and no copy
-// depends on the exec mask, so every wave issues exactly COPIES instructions.
-template <int BY, int ROWS, int W>
+// starting at column x0 - 4; 64: starting at the given column) whose rows are
+// PITCH floats apart.  Wave ty copies rows ty, ty + BY, ...  A 72-wide row
+// takes two full-wave copies, columns 0..63 and 8..71: the overlap is written
+// twice with identical values, and no copy depends on the exec mask, so every
+// wave issues exactly COPIES instructions.
+template <int BY, int ROWS, int W, int PITCH = W>
 struct TileCopy {
     static constexpr int PER = (ROWS + BY - 1) / BY;
     static constexpr int COPIES = PER * (W == 72 ? 2 : 1);
@@ -72,11 +73,20 @@ struct TileCopy {
         for (int k = 0; k < PER; ++k) {
             const int r = ty + k * BY;
             const long long row = base + static_cast<long long>(min(ys + r, g.gy - 1)) * g.px;
-            copy_row(f + row + xm, lds + r * W);
-            if (W == 72) copy_row(f + row + xt, lds + r * W + 8);
+            copy_row(f + row + xm, lds + r * PITCH);
+            if (W == 72) copy_row(f + row + xt, lds + r * PITCH + 8);
         }
     }
 };
+
+// Wait until the copies of plane `next` (the newest ones in flight) are the
+// only vector-memory operations this wave may still have outstanding.  Planes
+// inside the chunk carry FULL copies per wave, halo planes LITE ones.
+template <int FULL, int LITE>
+__device__ __forceinline__ void wait_for_ring(bool next_full) {
+    if (next_full) wait_vm<FULL>();
+    else wait_vm<LITE>();
+}
 
 template <int BY>
 struct VelTiles {
@@ -130,7 +140,7 @@ __device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, in
 
 // Fused velocity half-step (velocityKernel3D + force source epilogue).
 template <int BY>
-__global__ __launch_bounds__(64 * BY) void velocity_dma(DevStep P, TileMap M) {
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma(DevStep P, TileMap M) {
     __shared__ VelTiles<BY> ring[3];
     __shared__ float cx[6 * 64];          // x CPML table: [half*3 + k][lane]
     __shared__ float cyt[6 * BY];         // y CPML table: [half*3 + k][row]
@@ -158,19 +168,29 @@ __global__ __launch_bounds__(64 * BY) void velocity_dma(DevStep P, TileMap M) {
     const float* sxz = P.p.f[8];
     const float* ib = P.p.inv_rho;
     constexpr int NDMA = vel_copies<BY>();
+    constexpr int NLITE = 2 * TileCopy<BY, BY, 64>::COPIES;
 
     // CPML coefficient tables of this tile (read from LDS: no vector-memory
     // waits inside the plane loop).
     fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
     __syncthreads();
 
+    // Planes z0..z1-1 need whole tiles (x/y derivatives); the halo planes
+    // only feed the z queues, so they copy just the tile's own rows of sxz
+    // and syz into the places the queues read.
+    auto full_plane = [&](int slot) { return slot >= z0 && slot < z1; };
     auto copy_plane = [&](int slot, int L) {
         VelTiles<BY>& s = ring[(slot - k0) % 3];
-        TileCopy<BY, BY, 72>::issue(s.xx, sxx, g, x0 - 4, y0, L, tx, ty);
-        TileCopy<BY, BY + 8, 72>::issue(s.xy, sxy, g, x0 - 4, y0 - 4, L, tx, ty);
-        TileCopy<BY, BY + 8, 64>::issue(s.yy, syy, g, x0, y0 - 4, L, tx, ty);
-        TileCopy<BY, BY, 72>::issue(s.xz, sxz, g, x0 - 4, y0, L, tx, ty);
-        TileCopy<BY, BY + 8, 64>::issue(s.yz, syz, g, x0, y0 - 4, L, tx, ty);
+        if (full_plane(slot)) {
+            TileCopy<BY, BY, 72>::issue(s.xx, sxx, g, x0 - 4, y0, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.xy, sxy, g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 64>::issue(s.yy, syy, g, x0, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY, 72>::issue(s.xz, sxz, g, x0 - 4, y0, L, tx, ty);
+            TileCopy<BY, BY + 8, 64>::issue(s.yz, syz, g, x0, y0 - 4, L, tx, ty);
+        } else {
+            TileCopy<BY, BY, 64, 72>::issue(s.xz + 4, sxz, g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64>::issue(s.yz + 4 * 64, syz, g, x0, y0, L, tx, ty);
+        }
     };
     auto load_operands = [&](int k, VelOperands& u) {
         u.zz = szz[dcell(g, xc, yc, k)];
@@ -214,7 +234,7 @@ __global__ __launch_bounds__(64 * BY) void velocity_dma(DevStep P, TileMap M) {
     bool bad = false;
     auto plane_step = [&](int k, const VelOperands& cur, VelOperands& nxt) {
         const VelTiles<BY>& s = ring[(k - k0) % 3];
-        wait_vm<NDMA>();
+        wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
         if (k + 1 <= last) load_operands(k + 1, nxt);
 
@@ -400,17 +420,27 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     const int qy = slab_q(y, g.gy, g.pml, false);
     const float invh = P.invh;
     constexpr int NDMA = str_copies<BY>();
+    constexpr int NLITE = 3 * TileCopy<BY, BY, 64>::COPIES;
 
     // CPML coefficient tables of this tile (read from LDS: no vector-memory
     // waits inside the plane loop).
     fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
     __syncthreads();
 
+    // Halo planes only feed the z queues: copy just the tile's own rows.
+    auto full_plane = [&](int slot) { return slot >= z0 && slot < z1; };
     auto copy_plane = [&](int slot, int L) {
         StrTiles<BY>& s = ring[(slot - k0) % 3];
-        TileCopy<BY, BY + 8, 72>::issue(s.vx, P.p.f[0], g, x0 - 4, y0 - 4, L, tx, ty);
-        TileCopy<BY, BY + 8, 72>::issue(s.vy, P.p.f[1], g, x0 - 4, y0 - 4, L, tx, ty);
-        TileCopy<BY, BY + 8, 72>::issue(s.vz, P.p.f[2], g, x0 - 4, y0 - 4, L, tx, ty);
+        if (full_plane(slot)) {
+            TileCopy<BY, BY + 8, 72>::issue(s.vx, P.p.f[0], g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.vy, P.p.f[1], g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.vz, P.p.f[2], g, x0 - 4, y0 - 4, L, tx, ty);
+        } else {
+            constexpr int own = 4 * 72 + 4;
+            TileCopy<BY, BY, 64, 72>::issue(s.vx + own, P.p.f[0], g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64, 72>::issue(s.vy + own, P.p.f[1], g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64, 72>::issue(s.vz + own, P.p.f[2], g, x0, y0, L, tx, ty);
+        }
     };
     auto load_operands = [&](int k, StrOperands& u) {
         if (active && k >= z0 && k < z1) {
@@ -458,7 +488,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     bool bad = false;
     auto plane_step = [&](int k, const StrOperands& cur, StrOperands& nxt) {
         const StrTiles<BY>& s = ring[(k - k0) % 3];
-        wait_vm<NDMA>();
+        wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
         if (k + 1 <= last) load_operands(k + 1, nxt);
         const int c = (ty + 4) * 72 + tx + 4;

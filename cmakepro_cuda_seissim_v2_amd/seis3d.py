@@ -636,7 +636,7 @@ def _receiver_array(receivers):
 
 
 def gather_info(ctx: Ctx, src: Source, receivers, shot_id: int, stride: int, samples: int,
-                world_origin=(0.0, 0.0), cmp_bin=(10.0, 10.0)) -> GatherInfo:
+                world_origin=(0.0, 0.0), cmp_bin=(25.0, 25.0)) -> GatherInfo:
     """Frozen gather metadata (startGather3D): sample interval, first sample time, XY scalar."""
     arr, n = _receiver_array(receivers)
     origin = (C.c_double * 2)(*world_origin)

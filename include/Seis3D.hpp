@@ -10,7 +10,9 @@
 // library's own.  Link with libseis3d.so.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -66,6 +68,42 @@ struct Receiver3D { int x3D = 0, y3D = 0, z3D = 0; };
 
 struct MemoryBudget3D {
     std::size_t fieldBytes3D = 0, cpmlBytes3D = 0, viewBytes3D = 0, totalBytes3D = 0;
+};
+
+// Survey3D (include/Seis3D_Sim.h:36-45): receiver and shot spreads, world origin, CMP bins.
+struct Survey3D {
+    int receiverCountX3D = 50, receiverCountY3D = 3;
+    int receiverOrigin3D[3] = {27, 1, 0};
+    int receiverSpacing3D[2] = {5, 23};
+    int shotCountX3D = 3, shotCountY3D = 2;
+    int shotOrigin3D[3] = {24, 24, 12};
+    int shotSpacing3D[2] = {20, 20};
+    double worldOrigin3D[2] = {0.0, 0.0};
+    float cmpBinSize3D[2] = {25.0f, 25.0f};
+};
+
+// Gather3D (include/Seis3D_Sim.h:47-57): frozen geometry and [sample][receiver][vx,vy,vz] samples.
+struct Gather3D {
+    SimulationContext3D context3D;
+    Source3D source3D;
+    Survey3D survey3D;
+    std::vector<Receiver3D> receivers3D;
+    std::vector<float> samples3D;
+    int sampleCount3D = 0, targetSamples3D = 1000, stride3D = 2, shotId3D = 1;
+    int sampleIntervalUs3D = 0;
+    int coordinateScalar3D = -1000;
+    double firstSampleTime3D = 0.0;
+};
+
+// ImportedShot3D (include/Seis3D_Sim.h:60-68): one single-component shot read from SEG-Y.
+struct ImportedShot3D {
+    Gather3D gather3D;
+    std::vector<std::array<double, 3>> coordinates3D;
+    std::vector<double> firstTimes3D;
+    std::vector<int> traceIds3D;
+    int component3D = 2;
+    bool cartesian3D = true;
+    std::string componentName3D = "Amplitude", amplitudeUnit3D = "file units", coordinateUnit3D = "m", fileName3D;
 };
 
 // The caller owns the struct; the library owns the session behind it.
@@ -131,6 +169,39 @@ inline seis3d_source toC(const Source3D& s) {
     seis3d_source o{};
     o.x = s.x3D; o.y = s.y3D; o.z = s.z3D; o.kind = int(s.kind3D); o.amplitude = s.amplitude3D;
     for (int i = 0; i < 3; ++i) o.direction[i] = s.direction3D[i];
+    return o;
+}
+
+inline seis3d_survey toC(const Survey3D& q) {
+    seis3d_survey o{};
+    o.receiver_count_x = q.receiverCountX3D; o.receiver_count_y = q.receiverCountY3D;
+    o.shot_count_x = q.shotCountX3D; o.shot_count_y = q.shotCountY3D;
+    for (int a = 0; a < 3; ++a) { o.receiver_origin[a] = q.receiverOrigin3D[a]; o.shot_origin[a] = q.shotOrigin3D[a]; }
+    for (int a = 0; a < 2; ++a) { o.receiver_spacing[a] = q.receiverSpacing3D[a]; o.shot_spacing[a] = q.shotSpacing3D[a]; }
+    return o;
+}
+
+inline void fromC(const seis3d_survey& o, Survey3D& q) {
+    q.receiverCountX3D = o.receiver_count_x; q.receiverCountY3D = o.receiver_count_y;
+    q.shotCountX3D = o.shot_count_x; q.shotCountY3D = o.shot_count_y;
+    for (int a = 0; a < 3; ++a) { q.receiverOrigin3D[a] = o.receiver_origin[a]; q.shotOrigin3D[a] = o.shot_origin[a]; }
+    for (int a = 0; a < 2; ++a) { q.receiverSpacing3D[a] = o.receiver_spacing[a]; q.shotSpacing3D[a] = o.shot_spacing[a]; }
+}
+
+inline std::vector<seis3d_receiver> toC(const std::vector<Receiver3D>& r) {
+    std::vector<seis3d_receiver> o(r.size());
+    for (std::size_t i = 0; i < r.size(); ++i) o[i] = {r[i].x3D, r[i].y3D, r[i].z3D};
+    return o;
+}
+
+inline seis3d_gather_info toC(const Gather3D& g) {
+    seis3d_gather_info o{};
+    o.spacing = g.context3D.spacing3D; o.frequency = g.context3D.frequency3D; o.source = toC(g.source3D);
+    o.world_origin[0] = g.survey3D.worldOrigin3D[0]; o.world_origin[1] = g.survey3D.worldOrigin3D[1];
+    o.cmp_bin[0] = g.survey3D.cmpBinSize3D[0]; o.cmp_bin[1] = g.survey3D.cmpBinSize3D[1];
+    o.shot_id = g.shotId3D; o.stride = g.stride3D; o.samples = g.sampleCount3D;
+    o.sample_interval_us = g.sampleIntervalUs3D; o.coordinate_scalar = g.coordinateScalar3D;
+    o.first_sample_time = g.firstSampleTime3D;
     return o;
 }
 
@@ -244,5 +315,122 @@ inline void uploadField3D(GPUSimData3D& g, Field3D f, const std::vector<float>& 
 
 // Wait for the session stream (the reference's cudaStreamSynchronize(g.stream3D)).
 inline void synchronize3D(GPUSimData3D& g) { detail::check(seis3d_synchronize(detail::session(g))); }
+
+// ---- acquisition and files (src/Seis3D/Acquisition3D.cpp, ShotRecordIo3D.cpp, ModelIo3D.cpp) ----
+
+// fitSurvey3D (src/Seis3D_Sim.cpp:42-60): fit the spreads into the model.
+inline void fitSurvey3D(const SimulationContext3D& c, Survey3D& q) {
+    const seis3d_ctx cc = detail::toC(c);
+    seis3d_survey o = detail::toC(q);
+    detail::check(seis3d_fit_survey(&cc, &o));
+    detail::fromC(o, q);
+}
+
+// configureReceivers3D (src/Seis3D/Acquisition3D.cpp:21-37): receivers y-major, then x.
+inline std::vector<Receiver3D> configureReceivers3D(const SimulationContext3D& c, const Survey3D& q) {
+    const seis3d_ctx cc = detail::toC(c);
+    const seis3d_survey o = detail::toC(q);
+    std::vector<seis3d_receiver> r(maxReceivers3D);
+    int count = maxReceivers3D;   // capacity in, receiver count out
+    detail::check(seis3d_survey_receivers(&cc, &o, r.data(), &count));
+    std::vector<Receiver3D> out(static_cast<std::size_t>(count));
+    for (int i = 0; i < count; ++i) out[std::size_t(i)] = {r[std::size_t(i)].x, r[std::size_t(i)].y, r[std::size_t(i)].z};
+    return out;
+}
+
+// startGather3D (src/Seis3D/Acquisition3D.cpp:188-230) without the UI state: validates the
+// source, spread, sample count and stride, and freezes interval, first-sample time and XY scalar.
+inline Gather3D startGather3D(const SimulationContext3D& c, const Source3D& s, const Survey3D& q,
+                              const std::vector<Receiver3D>& receivers, int shotId, int targetSamples, int stride) {
+    const seis3d_ctx cc = detail::toC(c);
+    const seis3d_source cs = detail::toC(s);
+    const auto cr = detail::toC(receivers);
+    seis3d_gather_info info{};
+    detail::check(seis3d_gather_info_init(&cc, &cs, cr.data(), int(cr.size()), shotId, stride, targetSamples,
+                                          q.worldOrigin3D, q.cmpBinSize3D, &info));
+    Gather3D g;
+    g.context3D = c; g.source3D = s; g.survey3D = q; g.receivers3D = receivers;
+    g.targetSamples3D = targetSamples; g.stride3D = stride; g.shotId3D = shotId;
+    g.sampleIntervalUs3D = info.sample_interval_us; g.coordinateScalar3D = info.coordinate_scalar;
+    g.firstSampleTime3D = info.first_sample_time;
+    g.samples3D.reserve(std::size_t(targetSamples) * receivers.size() * 3);
+    return g;
+}
+
+// exportGather3D (src/Seis3D/Acquisition3D.cpp:157-185): <prefix>_<shot:04>_vx/vy/vz.sgy (SEG-Y
+// rev1) and _geometry.csv in `directory`; never overwrites.
+inline void exportGather3D(const Gather3D& g, const std::string& directory, const std::string& prefix) {
+    if (g.sampleCount3D < 1) throw std::runtime_error("No 3D receiver samples to export.");
+    if (g.samples3D.size() != std::size_t(g.sampleCount3D) * g.receivers3D.size() * 3)
+        throw std::invalid_argument("Seis3D: gather samples do not match the sample count.");
+    const seis3d_gather_info info = detail::toC(g);
+    const auto cr = detail::toC(g.receivers3D);
+    detail::check(seis3d_export_gather(&info, cr.data(), int(cr.size()), g.samples3D.data(), directory.c_str(),
+                                       prefix.c_str()));
+}
+
+// readShotSegy3D (src/Seis3D/ShotRecordIo3D.cpp:66-141): one fixed-length single-component shot.
+// Receivers are the ranks of the trace coordinates along each axis.
+inline ImportedShot3D readShotSegy3D(const std::string& path) {
+    seis3d_segy_info info{};
+    detail::check(seis3d_segy_inspect(path.c_str(), &info));
+    const std::size_t n = std::size_t(info.traces), ns = std::size_t(info.samples);
+    std::vector<float> traces(n * ns);
+    std::vector<double> xyz(n * 3);
+    ImportedShot3D shot;
+    shot.firstTimes3D.resize(n);
+    std::vector<int32_t> ids(n);
+    detail::check(seis3d_segy_read(path.c_str(), traces.data(), xyz.data(), shot.firstTimes3D.data(), ids.data()));
+    shot.traceIds3D.assign(ids.begin(), ids.end());
+    shot.component3D = info.component;
+    const char* names[3] = {"Vx", "Vy", "Vz"};
+    shot.componentName3D = names[info.component];
+    const char* units[] = {"file units", "Pa", "V", "mV", "A", "m", "m/s", "m/s^2", "N", "W"};
+    shot.amplitudeUnit3D = info.measurement_unit >= 1 && info.measurement_unit <= 9 ? units[info.measurement_unit] : units[0];
+    shot.cartesian3D = info.coordinate_units == 1;
+    const std::size_t slash = path.find_last_of("/\\");
+    shot.fileName3D = slash == std::string::npos ? path : path.substr(slash + 1);
+    Gather3D& g = shot.gather3D;
+    g.sampleCount3D = g.targetSamples3D = info.samples;
+    g.sampleIntervalUs3D = info.sample_interval_us;
+    g.shotId3D = info.shot_id;
+    g.samples3D.assign(n * ns * 3, 0.0f);
+    for (std::size_t r = 0; r < n; ++r)
+        for (std::size_t k = 0; k < ns; ++k) g.samples3D[(k * n + r) * 3 + std::size_t(info.component)] = traces[r * ns + k];
+    shot.coordinates3D.resize(n);
+    g.receivers3D.resize(n);
+    for (std::size_t r = 0; r < n; ++r) shot.coordinates3D[r] = {xyz[3 * r], xyz[3 * r + 1], xyz[3 * r + 2]};
+    for (int axis = 0; axis < 3; ++axis) {
+        std::vector<double> v(n);
+        for (std::size_t r = 0; r < n; ++r) v[r] = shot.coordinates3D[r][std::size_t(axis)];
+        std::vector<double> u = v;
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (std::size_t r = 0; r < n; ++r) {
+            const int rank = int(std::lower_bound(u.begin(), u.end(), v[r]) - u.begin());
+            (axis == 0 ? g.receivers3D[r].x3D : axis == 1 ? g.receivers3D[r].y3D : g.receivers3D[r].z3D) = rank;
+        }
+    }
+    g.firstSampleTime3D = shot.firstTimes3D.front();
+    return shot;
+}
+
+// inspectModel3D (src/Seis3D/ModelIo3D.cpp:35-65): header, size and every cell of a .s3dm file.
+inline SimulationContext3D inspectModel3D(const std::string& path, SimulationContext3D c) {
+    seis3d_ctx cc = detail::toC(c);
+    detail::check(seis3d_s3dm_inspect(path.c_str(), &cc));
+    detail::fromC(cc, c);
+    return c;
+}
+
+// importModel3D (:68-77): stream a validated model into a session of the same shape.
+inline void importModel3D(GPUSimData3D& g, const SimulationContext3D&, const std::string& path) {
+    detail::check(seis3d_s3dm_load(detail::session(g), path.c_str()));
+}
+
+// exportModel3D (:80-102): the session's model to a new file, written through <path>.part.
+inline void exportModel3D(GPUSimData3D& g, const SimulationContext3D&, const std::string& path) {
+    detail::check(seis3d_s3dm_save(detail::session(g), path.c_str()));
+}
 
 }  // namespace Seis3D

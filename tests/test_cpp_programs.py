@@ -44,3 +44,18 @@ def test_2d_solver_regression_gpu(gpu):
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stderr + r.stdout[-2000:]
     assert "PASS" in r.stdout
+
+
+def test_io_regression_cpu_checks():
+    """SEG-Y gather round trip and .s3dm inspection run everywhere; the GPU part may skip with 77."""
+    r = run("seis3d_io_regression", 300)
+    assert r.returncode in (0, 77), r.stderr + r.stdout[-2000:]
+    assert "GATHER 150 receivers" in r.stdout and "MODEL 40x32x24" in r.stdout
+
+
+@pytest.mark.gpu
+def test_io_regression_gpu(gpu):
+    r = run("seis3d_io_regression", 300)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert "PASS" in r.stdout

@@ -1,7 +1,8 @@
 """Model and shot-record files through live GPU sessions.
 
-* exportModel3D -> inspectModel3D -> importModel3D: the downloaded planes and 80 steps of the
-  solver on the imported model are bit-identical to the session the file came from.
+* exportModel3D writes exactly the planes downloadModelPlane3D returns. Those are Vp, Vs, rho
+  rebuilt from the stored 1/rho, lambda, mu (cuda/Cuda_Seis3D.cu:720-733), so a reload matches
+  the source session to a few ulp (not bit for bit), and 80 steps on it agree to rel-L2 1e-5.
 * A gather recorded on the GPU, exported to SEG-Y and read back, is bit-identical per component.
 """
 import numpy as np
@@ -33,15 +34,20 @@ def test_model_file_round_trip(gpu, tmp_path):
         c = S.inspectModel3D(path, ctx)
         assert (c.nx, c.ny, c.nz, c.free_surface, c.max_vp) == (40, 36, 32, 1, vp.max())
         assert c.dt == ctx.dt
+        payload = np.fromfile(path, "<f4", offset=36).reshape(32, 36, 40, 3)
+        for z in range(32):
+            assert np.array_equal(payload[z], a.download_plane(z))
         with S.GPUSimData3D(c) as b:
             S.importModel3D(b, path)
             for z in (0, 13, 31):
-                assert np.array_equal(a.download_plane(z), b.download_plane(z))
+                pa, pb = a.download_plane(z), b.download_plane(z)
+                assert np.max(np.abs(pa - pb) / np.abs(pa)) < 1e-6
             for k in range(80):
                 a.step(src, k)
                 b.step(src, k)
             for f in (S.VX, S.VZ, S.SXX, S.SXZ):
-                assert np.array_equal(a.download_field(f), b.download_field(f))
+                fa, fb = a.download_field(f), b.download_field(f)
+                assert np.linalg.norm(fa - fb) <= 1e-5 * np.linalg.norm(fa)
         wrong = W.imported_context(40, 36, 33, max_vp=float(vp.max()), pml=10)
         with S.GPUSimData3D(wrong) as w, pytest.raises(S.InvalidArgument, match="dimensions"):
             w.load_model(path)
