@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 LIBRARIES = {
-    "libseis2d.so": ["seis2d_kernels.hip", "seis2d_api.cpp"],
+    "libseis2d.so": ["seis2d_kernels.hip", "seis2d_api.cpp", "seis2d_io.cpp"],
     "libseis3d.so": ["seis3d_kernels.hip", "seis3d_pipe.hip", "seis3d_column.hip", "seis3d_dma.hip", "seis3d_adjoint.hip",
                     "seis3d_abc.hip", "seis3d_api.cpp", "seis3d_fwi.cpp", "seis3d_abc.cpp", "seis3d_io.cpp"],
 }
@@ -75,7 +75,7 @@ def build_oracle() -> Path:
 
 
 CPP_TESTS = {"seis3d_boundary_regression": "seis3d", "seis2d_solver_regression": "seis2d",
-             "seis3d_io_regression": "seis3d"}
+             "seis3d_io_regression": "seis3d", "seis2d_record_regression": "seis2d"}
 
 
 def build_cpp_tests() -> list[Path]:

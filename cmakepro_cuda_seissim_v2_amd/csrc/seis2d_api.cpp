@@ -17,6 +17,9 @@
 
 #include "../../include/seis2d.h"
 #include "seis2d_device.hpp"
+#include "seis2d_host.hpp"
+
+using namespace seis2d::host;
 
 namespace {
 
@@ -25,30 +28,8 @@ constexpr int kMaxSources = 200;           // SolverNumerics.h:10
 constexpr float kSourceScale = 1e7f;       // SolverNumerics.h:12
 constexpr int kChunk = 64;                 // steps per captured graph
 
-thread_local std::string g_error;
-
-struct Failure {
-    int code;
-    std::string message;
-};
-[[noreturn]] void fail(int code, const std::string& m) { throw Failure{code, m}; }
-[[noreturn]] void invalid(const std::string& m) { fail(SEIS2D_EINVAL, m); }
 void check(hipError_t e, const char* op) {
     if (e != hipSuccess) fail(SEIS2D_EHIP, std::string("HIP failure: ") + op + " | " + hipGetErrorString(e));
-}
-
-template <class F>
-int guarded(F&& fn) {
-    try {
-        fn();
-        return SEIS2D_OK;
-    } catch (const Failure& f) {
-        g_error = f.message;
-        return f.code;
-    } catch (const std::exception& e) {
-        g_error = e.what();
-        return SEIS2D_EHIP;
-    }
 }
 
 float courant_limit() {

@@ -63,6 +63,36 @@ class Source2D(C.Structure):
 _LIB = None
 
 
+class TraceMeta(C.Structure):
+    """seis2d_trace_meta: SeismicIO::TraceMetadata (field record, trace, source/group X and Z, scalar)."""
+    _fields_ = [("field_record", C.c_int32), ("trace_in_record", C.c_int32), ("source_x", C.c_int32),
+                ("source_y", C.c_int32), ("group_x", C.c_int32), ("group_y", C.c_int32),
+                ("coordinate_scalar", C.c_int32)]
+
+
+class RecordInfo(C.Structure):
+    _fields_ = [("traces", C.c_int32), ("samples", C.c_int32), ("sample_interval_us", C.c_int32),
+                ("format", C.c_int32), ("velocity_units", C.c_int32)]
+
+
+class FrozenModel(C.Structure):
+    """seis2d_frozen_model: FrozenModel2D, the solver inputs archived with a shot."""
+    _fields_ = [("nx", C.c_int32), ("nz", C.c_int32), ("npml", C.c_int32), ("formulation", C.c_int32),
+                ("source_x", C.c_int32), ("source_z", C.c_int32), ("source_type", C.c_int32),
+                ("capture_start_step", C.c_int32), ("sampling_stride", C.c_int32), ("free_surface", C.c_int32),
+                ("multiple_sources", C.c_int32), ("spacing", C.c_float), ("solver_dt", C.c_float),
+                ("frequency", C.c_float), ("delay", C.c_float), ("source_amplitude", C.c_float),
+                ("source_angle", C.c_float), ("lambda_", C.POINTER(C.c_float)), ("mu", C.POINTER(C.c_float)),
+                ("rho", C.POINTER(C.c_float)), ("wavelet", C.POINTER(C.c_float)), ("wavelet_count", C.c_int32)]
+
+
+class ExportRequest(C.Structure):
+    """seis2d_export_request: ShotExportRequest2D."""
+    _fields_ = [("directory", C.c_char_p), ("prefix", C.c_char_p), ("version", C.c_char_p), ("device", C.c_char_p),
+                ("shot", C.c_int32), ("samples", C.c_int32), ("sample_interval", C.c_float),
+                ("include_csv", C.c_int32), ("include_model", C.c_int32)]
+
+
 def library() -> C.CDLL:
     global _LIB
     if _LIB is None:
@@ -92,6 +122,13 @@ def library() -> C.CDLL:
             "seis2d_download_field": (C.c_int, [vp, C.c_int, _FP]),
             "seis2d_upload_field": (C.c_int, [vp, C.c_int, _FP]),
             "seis2d_last_error": (C.c_char_p, []),
+            "seis2d_write_segy": (C.c_int, [C.c_char_p, _FP, C.c_int, C.c_int, P(TraceMeta), C.c_float]),
+            "seis2d_read_segy_info": (C.c_int, [C.c_char_p, P(RecordInfo)]),
+            "seis2d_read_segy": (C.c_int, [C.c_char_p, _FP, P(C.c_double)]),
+            "seis2d_validate_prefix": (C.c_int, [C.c_char_p]),
+            "seis2d_sha256_file": (C.c_int, [C.c_char_p, C.c_char_p]),
+            "seis2d_export_shot_package": (C.c_int, [P(ExportRequest), P(FrozenModel), _FP, _FP, C.c_int,
+                                                     P(TraceMeta), C.c_char_p, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -347,6 +384,88 @@ def copyReceiverRecordsFromGpu(gpu: GPUSimData, num_rcv: int, nt: int, current_i
 
 
 # ---- setups ---------------------------------------------------------------------------------------
+
+# ---- shot records and shot packages (src/Seis2D/ShotExport2D.cpp, RecordData2D.cpp) ----
+
+def _meta_array(geometry):
+    arr = (TraceMeta * max(1, len(geometry)))()
+    for i, g in enumerate(geometry):
+        if isinstance(g, TraceMeta):
+            arr[i] = g
+        else:
+            arr[i] = TraceMeta(**g)
+    return arr
+
+
+def trace_metadata(receivers_xz, source_xz, spacing: float, shot: int = 1) -> list:
+    """Millimetre trace geometry of a capture (src/Seis2D/Acquisition2D.inc:61-73): scalar -1000,
+    Y holds Z."""
+    def mm(grid):
+        v = float(grid) * spacing * 1000
+        if not np.isfinite(v) or abs(v) > 2 ** 31 - 1:
+            raise InvalidArgument("Shot coordinate exceeds SEG-Y millimetre range.")
+        return int(np.round(v))
+    sx, sz = source_xz
+    return [TraceMeta(shot, i + 1, mm(sx), mm(sz), mm(x), mm(z), -1000) for i, (x, z) in enumerate(receivers_xz)]
+
+
+def writeSegyCaptureFile2D(traces, geometry, path, dt: float) -> None:
+    """One component [trace][sample] as SEG-Y, IEEE big-endian, m/s (SeismicIO.cpp:296-414)."""
+    a = _f32(np.asarray(traces, np.float32))
+    if a.ndim != 2 or a.shape[0] != len(geometry):
+        raise InvalidArgument("SEG-Y sample count/interval must fit 1..32767 and geometry must match.")
+    _check(library().seis2d_write_segy(str(path).encode(), _fp(a), a.shape[0], a.shape[1], _meta_array(geometry),
+                                       float(dt)))
+
+
+def readRecordSegy2D(path) -> dict:
+    """Traces [trace][sample], first-sample times, interval and units (RecordData2D.cpp:35-65)."""
+    info = RecordInfo()
+    _check(library().seis2d_read_segy_info(str(path).encode(), C.byref(info)))
+    traces = np.zeros((info.traces, info.samples), np.float32)
+    times = np.zeros(info.traces, np.float64)
+    _check(library().seis2d_read_segy(str(path).encode(), _fp(traces), times.ctypes.data_as(C.POINTER(C.c_double))))
+    return {"traces": traces, "first_times": times, "sample_interval": info.sample_interval_us * 1e-6,
+            "format": info.format, "units": "m/s" if info.velocity_units else "", "name": Path(path).name}
+
+
+def validateExportPrefix2D(prefix: str) -> None:
+    _check(library().seis2d_validate_prefix(prefix.encode()))
+
+
+def sha256_file(path) -> str:
+    out = C.create_string_buffer(65)
+    _check(library().seis2d_sha256_file(str(path).encode(), out))
+    return out.value.decode()
+
+
+def exportShotPackage2D(directory, vx, vz, geometry, frozen: dict, *, prefix: str = "seis2d_shot", shot: int = 1,
+                        sample_interval: float, version: str = "", device: str = "", include_csv: bool = True,
+                        include_model: bool = True) -> Path:
+    """Publish <directory>/<prefix>_<shot:04> (exportShotPackage2D); vx, vz are [trace][sample].
+
+    `frozen` holds FrozenModel2D's members (nx, nz, npml, formulation, source_x, source_z, source_type,
+    capture_start_step, sampling_stride, free_surface, multiple_sources, spacing, solver_dt, frequency,
+    delay, source_amplitude, source_angle, lambda_, mu, rho, wavelet)."""
+    ax, az = _f32(np.asarray(vx, np.float32)), _f32(np.asarray(vz, np.float32))
+    if ax.ndim != 2 or ax.shape != az.shape or ax.shape[0] != len(geometry):
+        raise InvalidArgument("Shot components and geometry differ.")
+    keep = {}
+    m = FrozenModel()
+    for k, v in frozen.items():
+        if k in ("lambda_", "mu", "rho", "wavelet"):
+            keep[k] = _f32(np.asarray(v, np.float32).reshape(-1))
+            setattr(m, k, _fp(keep[k]))
+        else:
+            setattr(m, k, v)
+    m.wavelet_count = int(keep["wavelet"].size) if "wavelet" in keep else 0
+    req = ExportRequest(str(directory).encode(), prefix.encode(), version.encode(), device.encode(), int(shot),
+                        int(ax.shape[1]), float(sample_interval), int(include_csv), int(include_model))
+    out = C.create_string_buffer(4096)
+    _check(library().seis2d_export_shot_package(C.byref(req), C.byref(m), _fp(ax), _fp(az), ax.shape[0],
+                                                _meta_array(geometry), out, 4096))
+    return Path(out.value.decode())
+
 
 def homogeneous_context(nx: int, nz: int, h: float, dt: float, nt: int, npml: int, vp: float, vs: float, rho: float,
                         flag_type: int = SPLIT_PML, up_flag: bool = False, f0: float = 25.0, t0: float | None = None,

@@ -13,6 +13,7 @@
 
 #include <array>
 #include <cstddef>
+#include <filesystem>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -191,3 +192,126 @@ inline void copyReceiverRecordsFromGpu(GPUSimData& g, int numRcv, int nt, int it
 inline void recordReceiverStepOnGpu(GPUSimData& g, int numRcv, float* vx, float* vz) {
     seis2d_detail::check(seis2d_record_receiver_step(g.session, numRcv, vx, vz));
 }
+
+// ---- shot records and shot packages ----
+
+namespace SeismicIO {
+
+// TraceMetadata (Main_header/SeismicIO.h:21-29).
+struct TraceMetadata {
+    int32_t fieldRecordNum = 1;
+    int32_t traceInRecord = 1;
+    int32_t sourceX = 0, sourceY = 0;
+    int32_t groupX = 0, groupY = 0;
+    int16_t coordinateScalar = 1;
+};
+
+inline std::vector<seis2d_trace_meta> toC(const std::vector<TraceMetadata>& m) {
+    std::vector<seis2d_trace_meta> o(m.size());
+    for (std::size_t i = 0; i < m.size(); ++i)
+        o[i] = {m[i].fieldRecordNum, m[i].traceInRecord, m[i].sourceX, m[i].sourceY, m[i].groupX, m[i].groupY,
+                m[i].coordinateScalar};
+    return o;
+}
+
+inline std::vector<float> flatten(const std::vector<std::vector<float>>& traces, std::size_t samples) {
+    std::vector<float> flat;
+    flat.reserve(traces.size() * samples);
+    for (const auto& t : traces) {
+        if (t.size() != samples) throw std::invalid_argument("SEG-Y export traces have different lengths.");
+        flat.insert(flat.end(), t.begin(), t.end());
+    }
+    return flat;
+}
+
+// writeSegyCaptureFile2D (Main_cpp/SeismicIO.cpp:296-414).
+inline void writeSegyCaptureFile2D(const std::vector<std::vector<float>>& data, const std::vector<TraceMetadata>& meta,
+                                   const std::filesystem::path& path, float dt) {
+    if (data.empty() || data.front().empty()) throw std::invalid_argument("SEG-Y export requires samples and a positive interval.");
+    if (meta.size() != data.size())
+        throw std::invalid_argument("SEG-Y sample count/interval must fit 1..32767 and geometry must match.");
+    const auto flat = flatten(data, data.front().size());
+    const auto m = toC(meta);
+    seis2d_detail::check(seis2d_write_segy(path.string().c_str(), flat.data(), int(data.size()),
+                                           int(data.front().size()), m.data(), dt));
+}
+
+}  // namespace SeismicIO
+
+namespace Seis2D {
+
+// RecordData2D of readRecordSegy2D (src/Seis2D/RecordData2D.cpp:35-65).
+struct RecordData2D {
+    std::vector<std::vector<float>> traces;
+    std::vector<double> firstTimes;
+    double sampleInterval = 0;
+    std::string name, units;
+};
+
+inline RecordData2D readRecordSegy2D(const std::filesystem::path& path) {
+    seis2d_record_info info{};
+    seis2d_detail::check(seis2d_read_segy_info(path.string().c_str(), &info));
+    std::vector<float> flat(std::size_t(info.traces) * std::size_t(info.samples));
+    RecordData2D r;
+    r.firstTimes.resize(std::size_t(info.traces));
+    seis2d_detail::check(seis2d_read_segy(path.string().c_str(), flat.data(), r.firstTimes.data()));
+    r.traces.resize(std::size_t(info.traces));
+    for (std::size_t c = 0; c < r.traces.size(); ++c)
+        r.traces[c].assign(flat.begin() + std::ptrdiff_t(c * std::size_t(info.samples)),
+                           flat.begin() + std::ptrdiff_t((c + 1) * std::size_t(info.samples)));
+    r.sampleInterval = info.sample_interval_us * 1e-6;
+    r.name = path.filename().string();
+    if (info.velocity_units) r.units = "m/s";
+    return r;
+}
+
+// FrozenModel2D and ShotExportRequest2D (Main_header/ShotExport2D.h:7-19).
+struct FrozenModel2D {
+    int nx = 0, nz = 0, npml = 0, formulation = 0, sourceX = 0, sourceZ = 0, sourceType = 0, captureStartStep = 0,
+        samplingStride = 1;
+    bool freeSurface = false, multipleSources = false;
+    float spacing = 0, solverDt = 0, frequency = 0, delay = 0, sourceAmplitude = 0, sourceAngle = 0;
+    std::vector<float> lambda, mu, rho, wavelet;
+};
+struct ShotExportRequest2D {
+    std::filesystem::path directory;
+    std::string prefix = "seis2d_shot", version, device;
+    int shot = 1, samples = 0;
+    float sampleInterval = 0;
+    bool includeCsv = true, includeModel = true;
+};
+
+// validateExportPrefix2D (src/Seis2D/ShotExport2D.cpp:103-114).
+inline void validateExportPrefix2D(const std::string& prefix) {
+    seis2d_detail::check(seis2d_validate_prefix(prefix.c_str()));
+}
+
+// exportShotPackage2D (src/Seis2D/ShotExport2D.cpp:117-187): the published folder.
+inline std::filesystem::path exportShotPackage2D(const ShotExportRequest2D& r, const FrozenModel2D& m,
+                                                 const std::vector<std::vector<float>>& vx,
+                                                 const std::vector<std::vector<float>>& vz,
+                                                 const std::vector<SeismicIO::TraceMetadata>& geometry) {
+    if (vx.empty() || vx.size() != vz.size() || vx.size() != geometry.size())
+        throw std::invalid_argument("Shot components and geometry differ.");
+    for (const auto* c : {&vx, &vz})
+        for (const auto& t : *c)
+            if (t.size() != std::size_t(r.samples)) throw std::invalid_argument("Shot sample counts differ.");
+    const auto fx = SeismicIO::flatten(vx, std::size_t(r.samples)), fz = SeismicIO::flatten(vz, std::size_t(r.samples));
+    const auto g = SeismicIO::toC(geometry);
+    const std::size_t cells = std::size_t(m.nx > 0 ? m.nx : 0) * std::size_t(m.nz > 0 ? m.nz : 0);
+    if (r.includeModel && (m.lambda.size() != cells || m.mu.size() != cells || m.rho.size() != cells))
+        throw std::invalid_argument("Frozen model arrays are absent, inconsistent or exceed 256 MiB.");
+    seis2d_frozen_model cm{m.nx, m.nz, m.npml, m.formulation, m.sourceX, m.sourceZ, m.sourceType, m.captureStartStep,
+                           m.samplingStride, m.freeSurface, m.multipleSources, m.spacing, m.solverDt, m.frequency,
+                           m.delay, m.sourceAmplitude, m.sourceAngle, m.lambda.data(), m.mu.data(), m.rho.data(),
+                           m.wavelet.data(), int32_t(m.wavelet.size())};
+    const std::string dir = r.directory.string();
+    seis2d_export_request cr{dir.c_str(), r.prefix.c_str(), r.version.c_str(), r.device.c_str(), r.shot, r.samples,
+                             r.sampleInterval, r.includeCsv, r.includeModel};
+    std::vector<char> out(4096);
+    seis2d_detail::check(seis2d_export_shot_package(&cr, &cm, fx.data(), fz.data(), int(vx.size()), g.data(),
+                                                    out.data(), out.size()));
+    return std::filesystem::path(out.data());
+}
+
+}  // namespace Seis2D

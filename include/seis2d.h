@@ -10,7 +10,8 @@
  * Every call returns 0 on success or an error code; seis2d_last_error()
  * gives the message (thread-local).  Codes: 1 invalid argument
  * (std::invalid_argument in the reference), 2 HIP failure (the reference's
- * CUDA_CHECK runtime_error), 3 lifecycle misuse.
+ * CUDA_CHECK runtime_error), 3 lifecycle misuse, 4 file runtime error
+ * (std::runtime_error: unreadable input, existing destination).
  */
 #ifndef SEIS2D_H
 #define SEIS2D_H
@@ -22,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { SEIS2D_OK = 0, SEIS2D_EINVAL = 1, SEIS2D_EHIP = 2, SEIS2D_ELIFECYCLE = 3 };
+enum { SEIS2D_OK = 0, SEIS2D_EINVAL = 1, SEIS2D_EHIP = 2, SEIS2D_ELIFECYCLE = 3, SEIS2D_EIO = 4 };
 
 /* Boundary schemes (SimulationContext::flag_type). */
 enum { SEIS2D_SPLIT_PML = 1, SEIS2D_HYBRID_PML = 2, SEIS2D_FREE_SURFACE = 3 };
@@ -142,6 +143,70 @@ int seis2d_download_field(seis2d_t* t, int field, float* out);
 int seis2d_upload_field(seis2d_t* t, int field, const float* in);
 
 const char* seis2d_last_error(void);
+
+/* ---- Shot records and shot packages (host only) ------------------------------ */
+
+/* SeismicIO::TraceMetadata (Main_header/SeismicIO.h:21-29): trace header geometry. */
+typedef struct seis2d_trace_meta {
+    int32_t field_record;       /* bytes 9-12 */
+    int32_t trace_in_record;    /* bytes 13-16 */
+    int32_t source_x, source_y; /* bytes 73-80 (Y holds Z, down) */
+    int32_t group_x, group_y;   /* bytes 81-88 */
+    int32_t coordinate_scalar;  /* bytes 71-72 (int16 on disk) */
+} seis2d_trace_meta;
+
+/* writeSegyCaptureFile2D (Main_cpp/SeismicIO.cpp:296-414): one component, traces
+ * [trace][sample], IEEE big-endian, m/s; every input is validated before the
+ * file is opened. */
+int seis2d_write_segy(const char* path, const float* traces, int n_traces, int n_samples,
+                      const seis2d_trace_meta* meta, float dt);
+
+/* readRecordSegy2D (src/Seis2D/RecordData2D.cpp:35-65). */
+typedef struct seis2d_record_info {
+    int32_t traces, samples, sample_interval_us, format;
+    int32_t velocity_units;     /* every trace has measurement unit 6 (m/s) */
+} seis2d_record_info;
+int seis2d_read_segy_info(const char* path, seis2d_record_info* info);
+/* traces [trace][sample]; first_times [trace] in seconds; either may be null. */
+int seis2d_read_segy(const char* path, float* traces, double* first_times);
+
+/* FrozenModel2D (Main_header/ShotExport2D.h:7-12): the solver inputs of a shot. */
+typedef struct seis2d_frozen_model {
+    int32_t nx, nz, npml, formulation, source_x, source_z, source_type, capture_start_step, sampling_stride;
+    int32_t free_surface, multiple_sources;
+    float spacing, solver_dt, frequency, delay, source_amplitude, source_angle;
+    const float *lambda, *mu, *rho;   /* [nz][nx] */
+    const float* wavelet;
+    int32_t wavelet_count;
+} seis2d_frozen_model;
+
+/* ShotExportRequest2D (Main_header/ShotExport2D.h:13-19). */
+typedef struct seis2d_export_request {
+    const char* directory;
+    const char* prefix;          /* default "seis2d_shot" */
+    const char* version;
+    const char* device;
+    int32_t shot, samples;
+    float sample_interval;
+    int32_t include_csv, include_model;
+} seis2d_export_request;
+
+/* validateExportPrefix2D (src/Seis2D/ShotExport2D.cpp:103-114). */
+int seis2d_validate_prefix(const char* prefix);
+
+/* exportShotPackage2D (src/Seis2D/ShotExport2D.cpp:117-187): a new folder
+ * <directory>/<prefix>_<shot:04> holding record_vx.sgy, record_vz.sgy,
+ * geometry.csv, optionally record_vz.csv, model.s2dm and source_wavelet.f32le,
+ * and manifest.json (schema SeisWaveShot2D-1) with each file's size and
+ * SHA-256.  Written in a staging folder and renamed; an existing destination
+ * is an error (code 4).  vx, vz are [trace][sample]; the folder path is copied
+ * to out_path. */
+int seis2d_export_shot_package(const seis2d_export_request* request, const seis2d_frozen_model* model,
+                               const float* vx, const float* vz, int traces, const seis2d_trace_meta* geometry,
+                               char* out_path, size_t out_capacity);
+
+/* SHA-256 (FIPS 180-4) of a file, as 64 lowercase hex digits plus NUL. */
+int seis2d_sha256_file(const char* path, char out_hex[65]);
 
 #ifdef __cplusplus
 }

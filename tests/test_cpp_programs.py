@@ -59,3 +59,10 @@ def test_io_regression_gpu(gpu):
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stderr + r.stdout[-2000:]
     assert "PASS" in r.stdout
+
+
+def test_2d_record_regression():
+    """Host-only: the 2D record and shot-package checks must pass everywhere."""
+    r = run("seis2d_record_regression", 300)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    assert "PASS" in r.stdout
