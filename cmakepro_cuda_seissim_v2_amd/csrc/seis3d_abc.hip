@@ -151,21 +151,24 @@ __device__ void gather_row(const Frame& F, int i, unsigned char code, int part, 
 }
 
 // Scale s = max |field feature| of each tile row; rows normalised into xs (BF16) / X (FP32).
+// A window quieter than kMinScale is silent: zero features and zero scale, so no
+// correction (1 / s would overflow for denormal s and turn zero taps into NaN).
 __device__ void normalise_tile(Lds& L, int wave, int rows, float* X, float* S, long long row0) {
     const int lane = int(threadIdx.x) & 63, cellr = lane >> 2, part = lane & 3;
     float mx = 0.0f;
     for (int f = part; f < 9 * kTaps; f += 4) mx = fmaxf(mx, fabsf(L.xf[wave][cellr][f]));
     mx = fmaxf(mx, __shfl_xor(mx, 1));
     mx = fmaxf(mx, __shfl_xor(mx, 2));
-    const float inv = mx > 0.0f ? 1.0f / mx : 0.0f;
+    const bool live = mx >= kMinScale;
+    const float inv = live ? 1.0f / mx : 0.0f;
     for (int f = part; f < kIn; f += 4) {
-        const float v = f < 9 * kTaps ? L.xf[wave][cellr][f] * inv : (mx > 0.0f ? L.xf[wave][cellr][f] : 0.0f);
+        const float v = f < 9 * kTaps ? L.xf[wave][cellr][f] * inv : (live ? L.xf[wave][cellr][f] : 0.0f);
         L.xs[wave][cellr][f] = to_bf16(v);
         if (X && cellr < rows) X[(row0 + cellr) * kIn + f] = v;
     }
     if (part == 0) {
-        L.scale[wave][cellr] = mx;
-        if (S && cellr < rows) S[row0 + cellr] = mx;
+        L.scale[wave][cellr] = live ? mx : 0.0f;
+        if (S && cellr < rows) S[row0 + cellr] = live ? mx : 0.0f;
     }
 }
 
@@ -232,10 +235,15 @@ __global__ __launch_bounds__(128) void features_kernel(Frame F, FrameCells C, fl
     for (int p = 0; p < 4; ++p) gather_row(F, i, code, p, row);
     float mx = 0.0f;
     for (int f = 0; f < 9 * kTaps; ++f) mx = fmaxf(mx, fabsf(row[f]));
-    const float inv = mx > 0.0f ? 1.0f / mx : 0.0f;
-    for (int f = 0; f < kIn; ++f)
-        X[static_cast<long long>(k) * kIn + f] = f < 9 * kTaps ? row[f] * inv : (mx > 0.0f ? row[f] : 0.0f);
-    S[k] = mx;
+    const bool live = mx >= kMinScale;
+    const float inv = live ? 1.0f / mx : 0.0f;
+    bool finite = true;
+    for (int f = 0; f < kIn; ++f) {
+        const float v = f < 9 * kTaps ? row[f] * inv : (live ? row[f] : 0.0f);
+        finite = finite && isfinite(v);
+        X[static_cast<long long>(k) * kIn + f] = v;
+    }
+    S[k] = live && finite ? mx : 0.0f;   // a zero scale keeps the row out of training samples
     for (int ch = 0; ch < 9; ++ch) B[static_cast<long long>(k) * kOut + ch] = row[ch * kTaps + 3];   // tap 3 = the cell
 }
 

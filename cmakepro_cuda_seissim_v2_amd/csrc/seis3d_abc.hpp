@@ -20,6 +20,8 @@
 namespace seis3d::abc {
 
 constexpr int kIn = 96, kHidden = 32, kOut = 9, kOutPad = 16, kTaps = 8, kChannels = 12;
+// Windows whose largest |field feature| is below this are silent (no normalisation, no correction).
+constexpr float kMinScale = 1e-30f;
 // FP32 packing of the parameters: W1 [96][32], b1 [32], W2 [32][32], b2 [32], W3 [32][9], b3 [9].
 constexpr int kW1 = 0, kB1 = kW1 + kIn * kHidden, kW2 = kB1 + kHidden, kB2 = kW2 + kHidden * kHidden,
               kW3 = kB2 + kHidden, kB3 = kW3 + kHidden * kOut, kParams = kB3 + kOut;

@@ -842,6 +842,85 @@ int seis3d_extract_slice(seis3d_t* t, int axis, int at, int component, float* ou
     });
 }
 
+int seis3d_make_view_layout(const seis3d_ctx* ctx, seis3d_view_layout* out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("Seis3D: null argument.");
+        const auto l = make_view_layout(*ctx);
+        seis3d_view_layout o{};
+        for (int a = 0; a < 3; ++a) {
+            o.slice_offset[a] = l.slice_offset[a];
+            o.domain_size[a] = l.domain[a];
+            o.domain_origin[a] = l.origin[a];
+        }
+        o.slice_width[0] = l.domain[0]; o.slice_height[0] = l.domain[2];
+        o.slice_width[1] = l.domain[1]; o.slice_height[1] = l.domain[2];
+        o.slice_width[2] = l.domain[0]; o.slice_height[2] = l.domain[1];
+        o.volume_offset = l.volume_offset;
+        o.total_samples = l.total;
+        o.volume_x = l.volume[0]; o.volume_y = l.volume[1]; o.volume_z = l.volume[2];
+        *out = o;
+    });
+}
+
+int seis3d_extract_views(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
+                         int material_component, int volume, int include_boundary, seis3d_view_stats* stats) {
+    return guarded([&] {
+        require_session(t);
+        const seis3d_ctx& c = t->c;
+        if (!out || !slices || material_component < 5 || material_component > 7 || component < 0 || component > 7 ||
+            slices[0] < 0 || slices[0] >= c.nx || slices[1] < 0 || slices[1] >= c.ny || slices[2] < 0 ||
+            slices[2] >= c.nz)
+            invalid("Seis3D: invalid view parameters.");
+        const auto l = make_view_layout(c);
+        const long long count = static_cast<long long>(volume ? l.total : l.volume_offset);
+        if (capacity < std::size_t(count) * 2) invalid("Seis3D: view buffer too small.");
+        seis3d::ViewParams vp{};
+        for (int a = 0; a < 3; ++a) {
+            vp.slice_offset[a] = static_cast<long long>(l.slice_offset[a]);
+            vp.volume[a] = l.volume[a];
+            vp.domain[a] = l.domain[a];
+            vp.origin[a] = l.origin[a];
+            vp.slices[a] = slices[a];
+        }
+        vp.volume_offset = static_cast<long long>(l.volume_offset);
+        vp.total = static_cast<long long>(l.total);
+        vp.nx = c.nx; vp.ny = c.ny; vp.nz = c.nz;
+        vp.ox = c.ox; vp.oy = c.oy; vp.oz = c.oz;
+        vp.component = component;
+        vp.material_component = material_component;
+        vp.include_boundary = include_boundary != 0;
+        float* d = nullptr;
+        unsigned* st = nullptr;
+        check(hipMalloc(&d, std::size_t(count) * 2 * sizeof(float)), "allocate view");
+        try {
+            check(hipMalloc(&st, 5 * sizeof(unsigned)), "allocate view statistics");
+            const unsigned init[5] = {0u, 0u, 0x7f7fffffu, 0u, 0u};
+            check(hipMemcpyAsync(st, init, sizeof(init), hipMemcpyHostToDevice, t->stream), "reset view statistics");
+            check(seis3d::launch_extract_views(t->base(), vp, count, d, st, t->stream), "kernel launch");
+            check(hipMemcpyAsync(out, d, std::size_t(count) * 2 * sizeof(float), hipMemcpyDeviceToHost, t->stream),
+                  "read view");
+            unsigned host[5]{};
+            check(hipMemcpyAsync(host, st, sizeof(host), hipMemcpyDeviceToHost, t->stream), "read diagnostics");
+            check(hipStreamSynchronize(t->stream), "finish view extraction");
+            (void)hipFree(d);
+            (void)hipFree(st);
+            d = nullptr;
+            st = nullptr;
+            if (host[1]) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; session stopped.");
+            if (stats) {
+                std::memcpy(&stats->maximum, host + 0, 4);
+                std::memcpy(&stats->material_min, host + 2, 4);
+                std::memcpy(&stats->material_max, host + 3, 4);
+                std::memcpy(&stats->velocity_maximum, host + 4, 4);
+            }
+        } catch (...) {
+            (void)hipFree(d);
+            (void)hipFree(st);
+            throw;
+        }
+    });
+}
+
 int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk) {
     return guarded([&] {
         require_session(t);
@@ -967,9 +1046,23 @@ std::vector<seis3d::KernelConfig> tile_candidates(const seis3d_ctx& c) {
     std::vector<seis3d::KernelConfig> out;
     out.push_back({0, 8, 32});
     const int uz = c.gz - 8;
-    for (int ty : {4, 8})
-        for (int zc : {8, 12, 16, 24, 32})
-            if (zc <= uz && zc <= 128) out.push_back({4, ty, zc});
+    for (int ty : {4, 8}) {
+        std::vector<int> chunks = {8, 12, 16, 24, 32};
+        // Chunk sizes whose launch fills whole rounds of resident workgroups
+        // (the last round of a launch that overflows a round runs mostly idle).
+        int per_cu = 0, cus = 0;
+        if (seis3d::dma_residency(ty, &per_cu, &cus) == hipSuccess && per_cu > 0 && cus > 0) {
+            const int layer = ((c.gx - 8 + 63) / 64) * ((c.gy - 8 + ty - 1) / ty);
+            for (int rounds = 1; rounds <= 6; ++rounds) {
+                const int nbz = rounds * per_cu * cus / layer;
+                if (nbz >= 1) chunks.push_back((uz + nbz - 1) / nbz);
+            }
+        }
+        std::sort(chunks.begin(), chunks.end());
+        chunks.erase(std::unique(chunks.begin(), chunks.end()), chunks.end());
+        for (int zc : chunks)
+            if (zc >= 8 && zc <= uz && zc <= 128) out.push_back({4, ty, zc});
+    }
     return out;
 }
 

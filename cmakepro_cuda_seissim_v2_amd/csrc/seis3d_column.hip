@@ -42,8 +42,8 @@ __global__ __launch_bounds__(64 * BY) void velocity_column(DevStep P, TileMap M)
     const DevGrid& g = P.g;
     const int x = 4 + bx * 64 + int(threadIdx.x), y = 4 + by * BY + int(threadIdx.y);
     if (x >= g.gx - 4 || y >= g.gy - 4) return;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const int qx = slab_q(x, g.gx, g.pml, false);
     const int qy = slab_q(y, g.gy, g.pml, false);
     const float invh = P.invh;
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(64 * BY) void stress_column(DevStep P, TileMap M) {
     const DevGrid& g = P.g;
     const int x = 4 + bx * 64 + int(threadIdx.x), y = 4 + by * BY + int(threadIdx.y);
     if (x >= g.gx - 4 || y >= g.gy - 4) return;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const int qx = slab_q(x, g.gx, g.pml, false);
     const int qy = slab_q(y, g.gy, g.pml, false);
     const float invh = P.invh;

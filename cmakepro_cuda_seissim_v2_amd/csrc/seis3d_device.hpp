@@ -148,6 +148,13 @@ __device__ __forceinline__ bool tile_of(const TileMap& m, int& bx, int& by, int&
     return true;
 }
 
+// Planes [z0, z1) of z-chunk bz: the updated depth gz - 8 split into nbz
+// chunks whose sizes differ by at most one (never more than zchunk planes).
+__device__ __forceinline__ void chunk_of(const TileMap& m, int gz, int bz, int& z0, int& z1) {
+    const int uz = gz - 8;
+    z0 = 4 + (bz * uz) / m.nbz;
+    z1 = 4 + ((bz + 1) * uz) / m.nbz;
+}
 
 template <int N>
 __device__ __forceinline__ void shift(float (&q)[N], float lead) {

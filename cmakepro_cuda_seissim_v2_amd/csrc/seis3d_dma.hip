@@ -18,6 +18,8 @@
 //    z derivatives come from 9-deep register queues of the tile's columns.
 // The loads are explicit waits rather than compiler-tracked, so every wait
 // is conservative: a wave may wait for more than it needs, never less.
+#include <algorithm>
+
 #include "seis3d_device.hpp"
 #include "seis3d_kernels.hpp"
 
@@ -106,7 +108,6 @@ constexpr int vel_copies() {
 // Per-cell operands of one velocity iteration (lead plane k, update plane k-4).
 struct VelOperands {
     float zz;          // szz at the lead plane, own column
-    float px[3], py[3];  // x- and y-slab CPML memories of the lead plane
     float v[3];        // velocities at the update plane
     float b[4];        // 1/rho at i, i+x, i+y, i+z
     float pz[3];       // z-slab CPML memories at the update plane
@@ -152,8 +153,8 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const int k0 = z0 - 4, last = z1 + 3;
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
@@ -194,16 +195,6 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
     };
     auto load_operands = [&](int k, VelOperands& u) {
         u.zz = szz[dcell(g, xc, yc, k)];
-        if (active && k >= z0 && k < z1) {
-            if (qx >= 0) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) u.px[s] = P.p.psi[0][psi_index_x(g, s, qx, y, k)];
-            }
-            if (qy >= 0) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) u.py[s] = P.p.psi[1][psi_index_y(g, s, qy, x, k)];
-            }
-        }
         const int z = k - 4;
         if (active && z >= z0) {
             const long long i = dcell(g, x, y, z);
@@ -219,11 +210,11 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
     };
 
     float q_xz[9], q_yz[9], q_zz[9];     // columns k-8..k (update plane z = k-4 in the middle)
-    float p_vx[5], p_vy[5], p_vz[5];     // CPML-filtered x+y sums of planes k-4..k
+    float p_vx[4], p_vy[4], p_vz[4];     // CPML-filtered x+y sums of planes k-4..k-1
 #pragma unroll
     for (int j = 0; j < 9; ++j) { q_xz[j] = 0.0f; q_yz[j] = 0.0f; q_zz[j] = 0.0f; }
 #pragma unroll
-    for (int j = 0; j < 5; ++j) { p_vx[j] = 0.0f; p_vy[j] = 0.0f; p_vz[j] = 0.0f; }
+    for (int j = 0; j < 4; ++j) { p_vx[j] = 0.0f; p_vy[j] = 0.0f; p_vz[j] = 0.0f; }
 
     // Two operand sets used alternately (the loop is unrolled by two), so the
     // prefetched registers are never copied while their loads are in flight.
@@ -237,75 +228,24 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
         wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
         if (k + 1 <= last) load_operands(k + 1, nxt);
+        // x/y CPML memories of plane k, used after the update below has run.
+        float px[3], py[3];
+        if (active && k >= z0 && k < z1) {
+            if (qx >= 0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) px[c] = P.p.psi[0][psi_index_x(g, c, qx, y, k)];
+            }
+            if (qy >= 0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) py[c] = P.p.psi[1][psi_index_y(g, c, qy, x, k)];
+            }
+        }
 
         shift(q_xz, s.xz[ty * 72 + tx + 4]);
         shift(q_yz, s.yz[(ty + 4) * 64 + tx]);
         shift(q_zz, cur.zz);
-        float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
-        if (active && k >= z0 && k < z1) {
-            float w[8];
-            // vx: D+x sxx (half), D-y sxy.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.xx[ty * 72 + tx + 1 + j];
-            float d = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[4 * 64 + tx] * cur.px[0] + cx[3 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 0, qx, y, k)] = m;
-                d = cx[5 * 64 + tx] * d + m;
-            }
-            pvx += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + j) * 72 + tx + 4];
-            d = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[1 * BY + ty] * cur.py[0] + cyt[0 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 0, qy, x, k)] = m;
-                d = cyt[2 * BY + ty] * d + m;
-            }
-            pvx += d;
-            // vy: D-x sxy, D+y syy (half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + 4) * 72 + tx + j];
-            d = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[1 * 64 + tx] * cur.px[1] + cx[0 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 1, qx, y, k)] = m;
-                d = cx[2 * 64 + tx] * d + m;
-            }
-            pvy += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.yy[(ty + 1 + j) * 64 + tx];
-            d = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[4 * BY + ty] * cur.py[1] + cyt[3 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 1, qy, x, k)] = m;
-                d = cyt[5 * BY + ty] * d + m;
-            }
-            pvy += d;
-            // vz: D-x sxz, D-y syz.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.xz[ty * 72 + tx + j];
-            d = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[1 * 64 + tx] * cur.px[2] + cx[0 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 2, qx, y, k)] = m;
-                d = cx[2 * 64 + tx] * d + m;
-            }
-            pvz += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.yz[(ty + j) * 64 + tx];
-            d = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[1 * BY + ty] * cur.py[2] + cyt[0 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 2, qy, x, k)] = m;
-                d = cyt[2 * BY + ty] * d + m;
-            }
-            pvz += d;
-        }
-        shift(p_vx, pvx);
-        shift(p_vy, pvy);
-        shift(p_vz, pvz);
-
+        // Update plane z = k - 4 first (its x/y sums are p_*[0]) so its stores
+        // drain while plane k's x/y derivatives are computed.
         const int z = k - 4;
         if (active && z >= z0) {
             const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
@@ -359,6 +299,70 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             P.p.f[0][i] = vx; P.p.f[1][i] = vy; P.p.f[2][i] = vz;
             bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
         }
+        float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
+        if (active && k >= z0 && k < z1) {
+            float w[8];
+            // vx: D+x sxx (half), D-y sxy.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xx[ty * 72 + tx + 1 + j];
+            float d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * px[0] + cx[3 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 0, qx, y, k)] = m;
+                d = cx[5 * 64 + tx] * d + m;
+            }
+            pvx += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + j) * 72 + tx + 4];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * py[0] + cyt[0 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 0, qy, x, k)] = m;
+                d = cyt[2 * BY + ty] * d + m;
+            }
+            pvx += d;
+            // vy: D-x sxy, D+y syy (half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xy[(ty + 4) * 72 + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * px[1] + cx[0 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 1, qx, y, k)] = m;
+                d = cx[2 * 64 + tx] * d + m;
+            }
+            pvy += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.yy[(ty + 1 + j) * 64 + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * py[1] + cyt[3 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 1, qy, x, k)] = m;
+                d = cyt[5 * BY + ty] * d + m;
+            }
+            pvy += d;
+            // vz: D-x sxz, D-y syz.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.xz[ty * 72 + tx + j];
+            d = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * px[2] + cx[0 * 64 + tx] * d;
+                P.p.psi[0][psi_index_x(g, 2, qx, y, k)] = m;
+                d = cx[2 * 64 + tx] * d + m;
+            }
+            pvz += d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.yz[(ty + j) * 64 + tx];
+            d = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * py[2] + cyt[0 * BY + ty] * d;
+                P.p.psi[1][psi_index_y(g, 2, qy, x, k)] = m;
+                d = cyt[2 * BY + ty] * d + m;
+            }
+            pvz += d;
+        }
+        shift(p_vx, pvx);
+        shift(p_vy, pvy);
+        shift(p_vz, pvz);
         // Last vector-memory instruction of the iteration: plane k+2 into the
         // buffer every wave finished reading before this iteration's barrier
         // (past the chunk end a clamped plane fills a buffer nobody reads).
@@ -389,7 +393,6 @@ constexpr int str_copies() {
 }
 
 struct StrOperands {
-    float px[3], py[3];  // x/y-slab CPML memories (slots 3-5) of the lead plane
     float o[6];          // stresses at the update plane
     float lam, mu;
     float r[7];          // 1/mu at i, i+x, i+y, i+z, i+x+y, i+y+z, i+x+z
@@ -412,8 +415,8 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const int k0 = z0 - 4, last = z1 + 3;
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int qx = slab_q(x, g.gx, g.pml, false);
@@ -443,16 +446,6 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
         }
     };
     auto load_operands = [&](int k, StrOperands& u) {
-        if (active && k >= z0 && k < z1) {
-            if (qx >= 0) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) u.px[s] = P.p.psi[0][psi_index_x(g, 3 + s, qx, y, k)];
-            }
-            if (qy >= 0) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) u.py[s] = P.p.psi[1][psi_index_y(g, 3 + s, qy, x, k)];
-            }
-        }
         const int z = k - 4;
         if (active && z >= z0) {
             const long long i = dcell(g, x, y, z);
@@ -473,11 +466,11 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     };
 
     float q_vx[9], q_vy[9], q_vz[9];
-    float p00[5], p11[5], p01[5], p20[5], p21[5];
+    float p00[4], p11[4], p01[4], p20[4], p21[4];  // x/y terms of planes k-4..k-1
 #pragma unroll
     for (int j = 0; j < 9; ++j) { q_vx[j] = 0.0f; q_vy[j] = 0.0f; q_vz[j] = 0.0f; }
 #pragma unroll
-    for (int j = 0; j < 5; ++j) { p00[j] = 0.0f; p11[j] = 0.0f; p01[j] = 0.0f; p20[j] = 0.0f; p21[j] = 0.0f; }
+    for (int j = 0; j < 4; ++j) { p00[j] = 0.0f; p11[j] = 0.0f; p01[j] = 0.0f; p20[j] = 0.0f; p21[j] = 0.0f; }
 
     // Two operand sets used alternately (the loop is unrolled by two), so the
     // prefetched registers are never copied while their loads are in flight.
@@ -491,68 +484,24 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
         wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
         if (k + 1 <= last) load_operands(k + 1, nxt);
+        // x/y CPML memories (slots 3-5) of plane k, used after the update below has run.
+        float px[3], py[3];
+        if (active && k >= z0 && k < z1) {
+            if (qx >= 0) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) px[j] = P.p.psi[0][psi_index_x(g, 3 + j, qx, y, k)];
+            }
+            if (qy >= 0) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) py[j] = P.p.psi[1][psi_index_y(g, 3 + j, qy, x, k)];
+            }
+        }
         const int c = (ty + 4) * 72 + tx + 4;
         shift(q_vx, s.vx[c]);
         shift(q_vy, s.vy[c]);
         shift(q_vz, s.vz[c]);
-        float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
-        if (active && k >= z0 && k < z1) {
-            float w[8];
-            // vx: D-x (slot 3, integer), D+y (slot 3, half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 4) * 72 + tx + j];
-            d00 = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[1 * 64 + tx] * cur.px[0] + cx[0 * 64 + tx] * d00;
-                P.p.psi[0][psi_index_x(g, 3, qx, y, k)] = m;
-                d00 = cx[2 * 64 + tx] * d00 + m;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 1 + j) * 72 + tx + 4];
-            float d01 = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[4 * BY + ty] * cur.py[0] + cyt[3 * BY + ty] * d01;
-                P.p.psi[1][psi_index_y(g, 3, qy, x, k)] = m;
-                d01 = cyt[5 * BY + ty] * d01 + m;
-            }
-            // vy: D+x (slot 4, half), D-y (slot 4, integer).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + 4) * 72 + tx + 1 + j];
-            float d10 = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[4 * 64 + tx] * cur.px[1] + cx[3 * 64 + tx] * d10;
-                P.p.psi[0][psi_index_x(g, 4, qx, y, k)] = m;
-                d10 = cx[5 * 64 + tx] * d10 + m;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + j) * 72 + tx + 4];
-            d11 = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[1 * BY + ty] * cur.py[1] + cyt[0 * BY + ty] * d11;
-                P.p.psi[1][psi_index_y(g, 4, qy, x, k)] = m;
-                d11 = cyt[2 * BY + ty] * d11 + m;
-            }
-            s01 = d01 + d10;
-            // vz: D+x (slot 5, half), D+y (slot 5, half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 4) * 72 + tx + 1 + j];
-            d20 = stencil8(w, invh);
-            if (qx >= 0) {
-                const float m = cx[4 * 64 + tx] * cur.px[2] + cx[3 * 64 + tx] * d20;
-                P.p.psi[0][psi_index_x(g, 5, qx, y, k)] = m;
-                d20 = cx[5 * 64 + tx] * d20 + m;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 1 + j) * 72 + tx + 4];
-            d21 = stencil8(w, invh);
-            if (qy >= 0) {
-                const float m = cyt[4 * BY + ty] * cur.py[2] + cyt[3 * BY + ty] * d21;
-                P.p.psi[1][psi_index_y(g, 5, qy, x, k)] = m;
-                d21 = cyt[5 * BY + ty] * d21 + m;
-            }
-        }
-        shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
-
+        // Update plane z = k - 4 first (its x/y terms are p*[0]) so its stores
+        // drain while plane k's x/y derivatives are computed.
         const int z = k - 4;
         if (active && z >= z0) {
             const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
@@ -608,6 +557,63 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) ||
                    !isfinite(sxz);
         }
+        float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
+        if (active && k >= z0 && k < z1) {
+            float w[8];
+            // vx: D-x (slot 3, integer), D+y (slot 3, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 4) * 72 + tx + j];
+            d00 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[1 * 64 + tx] * px[0] + cx[0 * 64 + tx] * d00;
+                P.p.psi[0][psi_index_x(g, 3, qx, y, k)] = m;
+                d00 = cx[2 * 64 + tx] * d00 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vx[(ty + 1 + j) * 72 + tx + 4];
+            float d01 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * py[0] + cyt[3 * BY + ty] * d01;
+                P.p.psi[1][psi_index_y(g, 3, qy, x, k)] = m;
+                d01 = cyt[5 * BY + ty] * d01 + m;
+            }
+            // vy: D+x (slot 4, half), D-y (slot 4, integer).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + 4) * 72 + tx + 1 + j];
+            float d10 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * px[1] + cx[3 * 64 + tx] * d10;
+                P.p.psi[0][psi_index_x(g, 4, qx, y, k)] = m;
+                d10 = cx[5 * 64 + tx] * d10 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vy[(ty + j) * 72 + tx + 4];
+            d11 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[1 * BY + ty] * py[1] + cyt[0 * BY + ty] * d11;
+                P.p.psi[1][psi_index_y(g, 4, qy, x, k)] = m;
+                d11 = cyt[2 * BY + ty] * d11 + m;
+            }
+            s01 = d01 + d10;
+            // vz: D+x (slot 5, half), D+y (slot 5, half).
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 4) * 72 + tx + 1 + j];
+            d20 = stencil8(w, invh);
+            if (qx >= 0) {
+                const float m = cx[4 * 64 + tx] * px[2] + cx[3 * 64 + tx] * d20;
+                P.p.psi[0][psi_index_x(g, 5, qx, y, k)] = m;
+                d20 = cx[5 * 64 + tx] * d20 + m;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = s.vz[(ty + 1 + j) * 72 + tx + 4];
+            d21 = stencil8(w, invh);
+            if (qy >= 0) {
+                const float m = cyt[4 * BY + ty] * py[2] + cyt[3 * BY + ty] * d21;
+                P.p.psi[1][psi_index_y(g, 5, qy, x, k)] = m;
+                d21 = cyt[5 * BY + ty] * d21 + m;
+            }
+        }
+        shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
         copy_plane(k + 2, min(k + 2, last));
     };
     int k = k0;
@@ -618,6 +624,24 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     if (k <= last) plane_step(k, opa, opb);
     wait_vm<0>();
     if (bad) atomicOr(P.p.flags, 1u);
+}
+
+hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {
+    int dev = 0, v = 0, st = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (tile_y == 4) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<4>, 64 * 4, 0);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<4>, 64 * 4, 0);
+    } else if (tile_y == 8) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<8>, 64 * 8, 0);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<8>, 64 * 8, 0);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    *blocks_per_cu = std::min(v, st);
+    return e;
 }
 
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {

@@ -186,8 +186,8 @@ __global__ __launch_bounds__(TX * BY) void velocity_stream(DevStep P, TileMap M)
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
     const int qx = slab_q(x, g.gx, g.pml, false);
@@ -340,8 +340,8 @@ __global__ __launch_bounds__(TX * BY) void stress_stream(DevStep P, TileMap M) {
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
     const int qx = slab_q(x, g.gx, g.pml, false);
@@ -656,6 +656,90 @@ __global__ void extract_slice(DevStep P, SliceParams sp, float* __restrict__ out
     if (!isfinite(value)) atomicOr(P.p.flags, 1u);
 }
 
+// Collocated scalar at a fractional padded position by trilinear weights
+// (resampleScalar3D, cuda/Cuda_Seis3D.cu:355-367); corners of zero weight are
+// not read.
+__device__ float resample_scalar(const DevStep& P, float fx, float fy, float fz, int component, float* magnitude) {
+    const int ix = int(floorf(fx)), iy = int(floorf(fy)), iz = int(floorf(fz));
+    const float fr[3] = {fx - ix, fy - iy, fz - iz};
+    float result = 0.0f;
+    if (magnitude) *magnitude = 0.0f;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const float w = (dx ? fr[0] : 1 - fr[0]) * (dy ? fr[1] : 1 - fr[1]) * (dz ? fr[2] : 1 - fr[2]);
+                if (w > 0) {
+                    const long long i = cell(P.g, ix + dx, iy + dy, iz + dz);
+                    result += w * sample_scalar(P, i, component);
+                    if (magnitude) *magnitude += w * sample_scalar(P, i, 3);
+                }
+            }
+    return result;
+}
+
+// extractKernel3D (cuda/Cuda_Seis3D.cu:370-418): sample i of the packed view ->
+// out[2i] (wave component), out[2i+1] (material component), with statistics.
+__global__ __launch_bounds__(256) void extract_views(DevStep P, ViewParams v, long long count, float* __restrict__ out,
+                                                     unsigned* __restrict__ stats) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < count;
+    float wave_max = 0.0f, vel_max = 0.0f, mat_min = 3.402823466e38f, mat_max = 0.0f;
+    bool bad = false;
+    if (valid) {
+        const int nx = v.domain[0], ny = v.domain[1], nz = v.domain[2];
+        int x, y, z;
+        float fx = 0, fy = 0, fz = 0;
+        const bool in_volume = i >= v.volume_offset;
+        if (i < v.slice_offset[1]) {
+            x = int(i % nx) + v.origin[0]; y = v.slices[1]; z = int(i / nx) + v.origin[2];
+        } else if (i < v.slice_offset[2]) {
+            const long long j = i - v.slice_offset[1];
+            x = v.slices[0]; y = int(j % ny) + v.origin[1]; z = int(j / ny) + v.origin[2];
+        } else if (!in_volume) {
+            const long long j = i - v.slice_offset[2];
+            x = int(j % nx) + v.origin[0]; y = int(j / nx) + v.origin[1]; z = v.slices[2];
+        } else {
+            const long long j = i - v.volume_offset;
+            fx = (float(j % v.volume[0]) + .5f) * float(nx) / v.volume[0] - .5f + v.origin[0];
+            fy = (float((j / v.volume[0]) % v.volume[1]) + .5f) * float(ny) / v.volume[1] - .5f + v.origin[1];
+            fz = (float(j / (static_cast<long long>(v.volume[0]) * v.volume[1])) + .5f) * float(nz) / v.volume[2] - .5f +
+                 v.origin[2];
+            x = int(floorf(fx)); y = int(floorf(fy)); z = int(floorf(fz));
+        }
+        float value, material, magnitude;
+        if (in_volume) {
+            value = resample_scalar(P, fx + v.ox, fy + v.oy, fz + v.oz, v.component, &magnitude);
+            material = resample_scalar(P, fx + v.ox, fy + v.oy, fz + v.oz, v.material_component, nullptr);
+        } else {
+            const long long c = cell(P.g, x + v.ox, y + v.oy, z + v.oz);
+            value = sample_scalar(P, c, v.component);
+            magnitude = sample_scalar(P, c, 3);
+            material = sample_scalar(P, c, v.material_component);
+        }
+        out[2 * i] = isfinite(value) ? value : 0.0f;
+        out[2 * i + 1] = isfinite(material) ? material : 0.0f;
+        bad = !isfinite(value) || !isfinite(material) || !isfinite(magnitude) || material < 0;
+        const bool physical = x >= 0 && x < v.nx && y >= 0 && y < v.ny && z >= 0 && z < v.nz;
+        if ((physical || v.include_boundary) && isfinite(value)) wave_max = fabsf(value);
+        if ((physical || v.include_boundary) && isfinite(magnitude)) vel_max = magnitude;
+        if (physical && isfinite(material)) { mat_min = material; mat_max = material; }
+    }
+    // Whole-wave reduction (every lane takes part; lanes past the end hold neutral values).
+    for (int off = 32; off > 0; off >>= 1) {
+        wave_max = fmaxf(wave_max, __shfl_xor(wave_max, off));
+        vel_max = fmaxf(vel_max, __shfl_xor(vel_max, off));
+        mat_min = fminf(mat_min, __shfl_xor(mat_min, off));
+        mat_max = fmaxf(mat_max, __shfl_xor(mat_max, off));
+    }
+    if (bad) atomicOr(stats + 1, 1u);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(stats + 0, __float_as_uint(wave_max));
+        atomicMin(stats + 2, __float_as_uint(mat_min));
+        atomicMax(stats + 3, __float_as_uint(mat_max));
+        atomicMax(stats + 4, __float_as_uint(vel_max));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -754,6 +838,13 @@ hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, h
 hipError_t launch_reciprocal_mu(const DevStep& P, hipStream_t s) {
     const long long cells = static_cast<long long>(P.g.gx) * P.g.gy * P.g.gz;
     hipLaunchKernelGGL(reciprocal_mu, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_views(const DevStep& P, const ViewParams& vp, long long count, float* out, unsigned* stats,
+                                hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(extract_views, dim3(unsigned((count + 255) / 256)), dim3(256), 0, s, P, vp, count, out, stats);
     return hipGetLastError();
 }
 

@@ -31,6 +31,20 @@ struct SliceParams {
     int component;
 };
 
+// Packed views (makeViewLayout3D / extractViews3D, cuda/Cuda_Seis3D.cu:370-418,486-501).
+struct ViewParams {
+    long long slice_offset[3], volume_offset, total;
+    int volume[3], domain[3], origin[3];
+    int slices[3];             // physical x, y, z of the three planes
+    int nx, ny, nz;            // physical extent (statistics cover physical cells unless include_boundary)
+    int ox, oy, oz;            // padded index of physical (0, 0, 0)
+    int component, material_component, include_boundary;
+};
+// stats: [0] max |wave|, [1] non-finite flag, [2] material min, [3] material max, [4] max |v|
+// (float bits of non-negative values, so unsigned ordering is float ordering).
+hipError_t launch_extract_views(const DevStep& P, const ViewParams& vp, long long count, float* out, unsigned* stats,
+                                hipStream_t s);
+
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s);
 // Software-pipelined 2.5D family (seis3d_pipe.hip), tile_y 4 or 8.
 hipError_t launch_pipe(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
@@ -38,6 +52,9 @@ hipError_t launch_pipe(bool velocity, const DevStep& P, int tile_y, int zchunk, 
 hipError_t launch_column(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
 // LDS-DMA staged 2.5D family (seis3d_dma.hip), tile_y 4 or 8.
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
+// Resident workgroups of the family-4 pair per CU (the smaller of velocity and
+// stress) and the device's CU count, for sizing launches in whole rounds.
+hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus);
 hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s);
 hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s);
 hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s);

@@ -99,8 +99,8 @@ __global__ __launch_bounds__(64 * BY) void velocity_pipe(DevStep P, TileMap M) {
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
     const int qx = slab_q(x, g.gx, g.pml, false);
@@ -257,8 +257,8 @@ __global__ __launch_bounds__(64 * BY) void stress_pipe(DevStep P, TileMap M) {
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = 4 + bz * M.zchunk;
-    const int z1 = min(z0 + M.zchunk, g.gz - 4);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
     const bool active = x < g.gx - 4 && y < g.gy - 4;
     const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
     const int qx = slab_q(x, g.gx, g.pml, false);

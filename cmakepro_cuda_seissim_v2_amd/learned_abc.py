@@ -31,9 +31,9 @@ from . import workloads as W
 N_PARAMS = 4457
 SHAPES = (("W1", (96, 32)), ("b1", (32,)), ("W2", (32, 32)), ("b2", (32,)), ("W3", (32, 9)), ("b3", (9,)))
 # Shipped weights, trained on one MI355X by this module's own trainer:
-#   python -m cmakepro_cuda_seissim_v2_amd.learned_abc train --runs 64 --size 64 --every 2 --epochs 30 \
+#   python -m cmakepro_cuda_seissim_v2_amd.learned_abc train --runs 200 --size 128 --epochs 40 \
 #       --out cmakepro_cuda_seissim_v2_amd/data/abc_weights.npy
-# (3.16 M teacher-forced samples from 64 random layered 64^3 runs; loss 6.9e-6 -> 5.4e-8).
+# (10.8 M teacher-forced samples from 200 seeded layered 128^3 runs; loss 7.1e-5 -> 1.75e-7).
 WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "abc_weights.npy"
 MARGIN = 8
 WIDE_PML = 40

@@ -119,6 +119,19 @@ class SegyInfo(C.Structure):
                 ("shot_id", C.c_int32), ("measurement_unit", C.c_int32), ("coordinate_units", C.c_int32)]
 
 
+class ViewLayout(C.Structure):
+    """seis3d_view_layout: ViewLayout3D sample counts of the packed view."""
+    _fields_ = [("slice_offset", C.c_uint64 * 3), ("slice_width", C.c_int32 * 3), ("slice_height", C.c_int32 * 3),
+                ("volume_offset", C.c_uint64), ("total_samples", C.c_uint64), ("volume_x", C.c_int32),
+                ("volume_y", C.c_int32), ("volume_z", C.c_int32), ("domain_size", C.c_int32 * 3),
+                ("domain_origin", C.c_int32 * 3)]
+
+
+class ViewStats(C.Structure):
+    _fields_ = [("maximum", C.c_float), ("material_min", C.c_float), ("material_max", C.c_float),
+                ("velocity_maximum", C.c_float)]
+
+
 class Timing(C.Structure):
     _fields_ = [("velocity_ms", C.c_double), ("stress_ms", C.c_double), ("total_ms", C.c_double),
                 ("launches", C.c_int64)]
@@ -173,6 +186,9 @@ def library() -> C.CDLL:
             "seis3d_upload_field": (C.c_int, [vp, C.c_int, P(C.c_float)]),
             "seis3d_extract_slice": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), C.c_size_t]),
             "seis3d_set_tiles": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+            "seis3d_make_view_layout": (C.c_int, [P(Ctx), P(ViewLayout)]),
+            "seis3d_extract_views": (C.c_int, [vp, P(C.c_float), C.c_size_t, P(C.c_int32), C.c_int, C.c_int, C.c_int,
+                                               C.c_int, P(ViewStats)]),
             "seis3d_get_tiles": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
             "seis3d_run_timed": (C.c_int, [vp, P(Source), C.c_uint64, C.c_int64, P(Timing)]),
             "seis3d_step_bytes": (C.c_int, [P(Ctx), P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
@@ -512,6 +528,19 @@ class GPUSimData3D:
         _check(library().seis3d_extract_slice(self._h, axis, at, component, _fptr(out), out.size))
         return out.reshape(dom[v], dom[u])
 
+    def extract_views(self, slices, component: int = 0, material_component: int = 5, volume: bool = True,
+                      include_boundary: bool = False):
+        """extractViews3D: ([sample][wave, material] array, statistics dict)."""
+        lay = make_view_layout(self.ctx)
+        n = lay.total_samples if volume else lay.volume_offset
+        out = np.zeros(2 * n, np.float32)
+        sl = (C.c_int32 * 3)(*[int(v) for v in slices])
+        st = ViewStats()
+        _check(library().seis3d_extract_views(self._h, _fptr(out), out.size, sl, int(component), int(material_component),
+                                              int(volume), int(include_boundary), C.byref(st)))
+        return out.reshape(n, 2), {"maximum": st.maximum, "material_range": (st.material_min, st.material_max),
+                                   "velocity_maximum": st.velocity_maximum}
+
     def set_tiles(self, family: int, tile_y: int = 8, z_chunk: int = 32) -> None:
         _check(library().seis3d_set_tiles(self._h, family, tile_y, z_chunk))
 
@@ -704,6 +733,13 @@ def write_model(path, ctx: Ctx, vp: np.ndarray, vs: np.ndarray, rho: np.ndarray)
     _check(library().seis3d_s3dm_write(os.fsencode(path), C.byref(ctx), _fptr(planes)))
 
 
+def make_view_layout(ctx: Ctx) -> ViewLayout:
+    """makeViewLayout3D: slices and volume of the display domain (CPML included, ghosts excluded)."""
+    out = ViewLayout()
+    _check(library().seis3d_make_view_layout(C.byref(ctx), C.byref(out)))
+    return out
+
+
 def device_count() -> int:
     return int(library().seis3d_device_count())
 
@@ -771,3 +807,12 @@ def importModel3D(g: GPUSimData3D, path) -> None:
 
 def exportModel3D(g: GPUSimData3D, path) -> None:
     g.save_model(path)
+
+
+def makeViewLayout3D(ctx: Ctx) -> ViewLayout:
+    return make_view_layout(ctx)
+
+
+def extractViews3D(g: GPUSimData3D, slices, component: int, material_component: int, volume: bool,
+                   include_boundary: bool = False):
+    return g.extract_views(slices, component, material_component, volume, include_boundary)

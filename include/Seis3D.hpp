@@ -106,6 +106,15 @@ struct ImportedShot3D {
     std::string componentName3D = "Amplitude", amplitudeUnit3D = "file units", coordinateUnit3D = "m", fileName3D;
 };
 
+// ViewLayout3D (include/Common3D.h:53-60).
+struct ViewLayout3D {
+    std::size_t sliceOffset3D[3]{};
+    int sliceWidth3D[3]{}, sliceHeight3D[3]{};
+    std::size_t volumeOffset3D = 0, totalSamples3D = 0;
+    int volumeX3D = 0, volumeY3D = 0, volumeZ3D = 0;
+    int domainSize3D[3]{}, domainOrigin3D[3]{};
+};
+
 // The caller owns the struct; the library owns the session behind it.
 struct GPUSimData3D {
     seis3d_t* session3D = nullptr;
@@ -315,6 +324,41 @@ inline void uploadField3D(GPUSimData3D& g, Field3D f, const std::vector<float>& 
 
 // Wait for the session stream (the reference's cudaStreamSynchronize(g.stream3D)).
 inline void synchronize3D(GPUSimData3D& g) { detail::check(seis3d_synchronize(detail::session(g))); }
+
+// makeViewLayout3D (cuda/Cuda_Seis3D.cu:486-501).
+inline ViewLayout3D makeViewLayout3D(const SimulationContext3D& c) {
+    const seis3d_ctx cc = detail::toC(c);
+    seis3d_view_layout o{};
+    detail::check(seis3d_make_view_layout(&cc, &o));
+    ViewLayout3D l;
+    for (int a = 0; a < 3; ++a) {
+        l.sliceOffset3D[a] = std::size_t(o.slice_offset[a]);
+        l.sliceWidth3D[a] = o.slice_width[a];
+        l.sliceHeight3D[a] = o.slice_height[a];
+        l.domainSize3D[a] = o.domain_size[a];
+        l.domainOrigin3D[a] = o.domain_origin[a];
+    }
+    l.volumeOffset3D = std::size_t(o.volume_offset);
+    l.totalSamples3D = std::size_t(o.total_samples);
+    l.volumeX3D = o.volume_x; l.volumeY3D = o.volume_y; l.volumeZ3D = o.volume_z;
+    return l;
+}
+
+// extractViews3D (cuda/Cuda_Seis3D.cu:627-646).  `out` is host memory here (the reference
+// writes a mapped GL buffer): 2 floats per sample, volume ? totalSamples3D : volumeOffset3D samples.
+inline void extractViews3D(GPUSimData3D& g, const SimulationContext3D&, float* out, const ViewLayout3D& layout,
+                           const int slices[3], int component, int materialComponent, bool volume, float& maximum,
+                           float materialRange[2], bool includeBoundary, float* velocityMaximum = nullptr) {
+    const std::size_t samples = volume ? layout.totalSamples3D : layout.volumeOffset3D;
+    const int32_t s[3] = {slices[0], slices[1], slices[2]};
+    seis3d_view_stats st{};
+    detail::check(seis3d_extract_views(detail::session(g), out, samples * 2, s, component, materialComponent, volume,
+                                       includeBoundary, &st));
+    maximum = st.maximum;
+    materialRange[0] = st.material_min;
+    materialRange[1] = st.material_max;
+    if (velocityMaximum) *velocityMaximum = st.velocity_maximum;
+}
 
 // ---- acquisition and files (src/Seis3D/Acquisition3D.cpp, ShotRecordIo3D.cpp, ModelIo3D.cpp) ----
 

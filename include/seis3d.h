@@ -215,6 +215,30 @@ int seis3d_upload_field(seis3d_t* t, int field, const float* in);
  * the display domain of makeViewLayout3D (CPML included, ghosts excluded). */
 int seis3d_extract_slice(seis3d_t* t, int axis, int at, int component, float* out, size_t capacity);
 
+/* ViewLayout3D (include/Common3D.h:53-60): sample counts of the packed view. */
+typedef struct seis3d_view_layout {
+    uint64_t slice_offset[3];
+    int32_t slice_width[3], slice_height[3];
+    uint64_t volume_offset, total_samples;
+    int32_t volume_x, volume_y, volume_z;
+    int32_t domain_size[3], domain_origin[3];
+} seis3d_view_layout;
+/* makeViewLayout3D (cuda/Cuda_Seis3D.cu:486-501). */
+int seis3d_make_view_layout(const seis3d_ctx* ctx, seis3d_view_layout* out);
+
+/* Statistics of extractViews3D: maximum |wave| (physical cells unless
+ * include_boundary), material range over physical cells, maximum |v|. */
+typedef struct seis3d_view_stats {
+    float maximum, material_min, material_max, velocity_maximum;
+} seis3d_view_stats;
+/* extractViews3D (cuda/Cuda_Seis3D.cu:627-646): the XZ, YZ and XY planes at
+ * slices[0..2] (physical x, y, z) and, if volume, the trilinear volume, as
+ * [sample][wave, material] floats in host memory (capacity in floats); wave
+ * component 0..7 as seis3d_extract_slice, material component 5..7.  A
+ * non-finite sample fails with code 4. */
+int seis3d_extract_views(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
+                         int material_component, int volume, int include_boundary, seis3d_view_stats* stats);
+
 /* ---- Tuning and measurement ---------------------------------------------- */
 
 /* Kernel family: 0 = direct (one thread per cell), 1 = 2.5D LDS/register
