@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--shots", type=int, default=0, help="shots of the batch (16) or fwi (8) workload")
     p.add_argument("--checkpoint", type=int, default=50, help="FWI checkpoint interval (steps)")
     p.add_argument("--fd", type=int, default=0, help="FWI: seeded central-difference directions to check")
+    p.add_argument("--no-copy-ceiling", action="store_true",
+                   help="skip the device-copy ceiling measurement (profiled runs)")
+    p.add_argument("--copy-ceiling", action="store_true",
+                   help="instead of the bench, print the measured device-copy bandwidth (GB/s, read + write)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="instead of the bench, run 5 plain device copies of MIB MiB (counter calibration)")
     return p.parse_args()
@@ -63,6 +67,41 @@ def copy_probe(mib: int) -> None:
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+
+
+def copy_ceiling_gbs(mib: int = 1024, reps: int = 10) -> float:
+    """Measured HBM ceiling: bytes read + written per second by a plain 1 GiB device copy.
+
+    Runs torch's device copy, so it must run in a process where the solver library has not
+    initialised the GPU (torch brings its own HIP runtime); see measured_copy_ceiling()."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / secs / 1e9
+
+
+def measured_copy_ceiling():
+    """copy_ceiling_gbs() in a child process (GB/s), or None if it fails."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-ceiling"], capture_output=True,
+                           text=True, timeout=300)
+        return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_ceiling_gbs"])
+    except Exception as e:  # the bench line still carries the paper peak
+        print(f"bench: copy ceiling unavailable ({e})", file=sys.stderr, flush=True)
+        return None
 
 
 def profiled_traffic(kernel: str, grid: str):
@@ -380,7 +419,8 @@ def bench_abc(args) -> None:
         g.release()
         print(f"bench: abc {case} {times[case]:.3f} ms/step", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    e = LA.reflection(params, model, src, steps, every=100)
+    e = LA.reflection(params, model, src, steps, every=100,
+                      log=lambda m: print(f"bench: abc {m}", file=sys.stderr, flush=True))
     secs = time.perf_counter() - t0
     print(f"bench: abc reflected energy {e} ({secs:.1f} s)", file=sys.stderr, flush=True)
     out = {
@@ -452,6 +492,9 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.copy_ceiling:
+        print(json.dumps({"copy_ceiling_gbs": copy_ceiling_gbs()}), flush=True)
+        return
     if args.workload == "batch":
         if args.steps == 1000:
             args.steps = 800
@@ -468,6 +511,9 @@ def main():
             args.steps = 1200
         fwi_bench(args)
         return
+
+    # Before this process touches the GPU: the copy ceiling runs in a child with torch's runtime.
+    ceiling = None if args.no_copy_ceiling else measured_copy_ceiling()
 
     from cmakepro_cuda_seissim_v2_amd import seis3d as S
     from cmakepro_cuda_seissim_v2_amd import workloads as W
@@ -554,9 +600,13 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if ceiling:
+        out["roofline"]["copy_ceiling_gbs"] = round(ceiling, 1)
+        out["roofline"]["frac_of_copy_ceiling"] = round(dominant[3] / ceiling, 4)
     tb, src_file = profiled_traffic(out["roofline"]["kernel"], args.grid)
     if tb:
         out["roofline"]["traffic"] = tb
+        out["roofline"]["traffic_gbs"] = round(tb / (dominant[1] * 1e-6) / 1e9, 1)
         out["roofline"]["traffic_source"] = f"profiles/{src_file} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)

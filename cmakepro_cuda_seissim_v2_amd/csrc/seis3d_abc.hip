@@ -204,7 +204,21 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
                 const unsigned char code = C.code[first + r];
                 const int axis = code >> 1;
                 const bool hi = (code & 1) != 0;
-                float v = out[j] * L.scale[wave][r];
+                // Dissipative correction: the network may only move a value towards zero
+                // (never grow it or change its sign), so the frame can remove energy but
+                // not add it.  Unrestricted one-step corrections feed back over long runs
+                // and grow without bound.
+                const float sc = L.scale[wave][r];
+                float d = out[j];
+                if (sc > 0.0f) {
+                    const float x = L.xf[wave][r][m * kTaps + 3] / sc;
+                    const float mag = fabsf(x);
+                    const float kept = fminf(fmaxf(copysignf(1.0f, x) * (x + d), 0.0f), mag);
+                    d = copysignf(kept, x) - x;
+                } else {
+                    d = 0.0f;
+                }
+                float v = d * sc;
                 if (m >= 3) v *= impedance(F, i);
                 if (!hi && kOdd[m] != 0.0f) v = -v;
                 delta[static_cast<long long>(first + r) * kOut + kFaceField[axis][m]] = v;

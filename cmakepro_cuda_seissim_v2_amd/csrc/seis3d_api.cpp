@@ -174,6 +174,13 @@ void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what)
     check(hipMemsetAsync(p, 0, count * sizeof(float), s), what);
 }
 
+void alloc_grid(float*& p, std::size_t cells, hipStream_t s, const char* what) {
+    float* raw = nullptr;
+    check(hipMalloc(&raw, (cells + 2 * kGridLead) * sizeof(float)), what);
+    check(hipMemsetAsync(raw, 0, (cells + 2 * kGridLead) * sizeof(float), s), what);
+    p = raw + kGridLead;
+}
+
 void upload_coefficients(seis3d_t* t, const seis3d_ctx& c, float* const dst[3]) {
     for (int a = 0; a < 3; ++a) {
         const auto v = make_coefficients(c, a);
@@ -450,11 +457,11 @@ int seis3d_create(const seis3d_ctx* ctx, seis3d_t** out) {
             if (free_bytes < reserve || t->plan.bytes > free_bytes - reserve)
                 fail(SEIS3D_EHIP, "Seis3D: device memory budget exceeded; reduce grid size or CPML thickness.");
             check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "create stream");
-            for (auto& f : t->fields) alloc_floats(f, t->plan.cells, t->stream, "allocate field");
-            alloc_floats(t->inv_rho, t->plan.cells, t->stream, "allocate field");
-            alloc_floats(t->lambda, t->plan.cells, t->stream, "allocate field");
-            alloc_floats(t->mu, t->plan.cells, t->stream, "allocate field");
-            alloc_floats(t->inv_mu, t->plan.cells, t->stream, "allocate field");
+            for (auto& f : t->fields) alloc_grid(f, t->plan.cells, t->stream, "allocate field");
+            alloc_grid(t->inv_rho, t->plan.cells, t->stream, "allocate field");
+            alloc_grid(t->lambda, t->plan.cells, t->stream, "allocate field");
+            alloc_grid(t->mu, t->plan.cells, t->stream, "allocate field");
+            alloc_grid(t->inv_mu, t->plan.cells, t->stream, "allocate field");
             if (!c.reflecting) {
                 for (int a = 0; a < 3; ++a) {
                     alloc_floats(t->psi[a], t->plan.slab[a] * 6, t->stream, "allocate CPML");

@@ -23,6 +23,16 @@ inline constexpr int kMaxVolumeOutput = 1536;                                   
 inline constexpr int kMaxReceivers = 300;                                         // Common3D.h:14
 inline constexpr int kMaxSegySamples = 32767;                                     // Common3D.h:15
 inline constexpr int kMaxSegyIntervalUs = 32767;                                  // Common3D.h:16
+
+// Grid arrays start kGridLead floats into their allocation, so that the first
+// interior column (x = 4) of every row is 128-byte aligned: row pitches are
+// multiples of 64 floats, and 28 + 4 = 32 floats = 128 bytes.  A 64-column
+// tile row is then exactly two cache lines.
+inline constexpr std::size_t kGridLead = 28;
+inline void free_grid(float*& p) noexcept {
+    if (p) (void)hipFree(p - kGridLead);
+    p = nullptr;
+}
 inline constexpr float kSourceNorm = 10.8387785335f;                              // (1 + 2 exp(-1/2))^3
 
 inline thread_local std::string g_error;
@@ -145,9 +155,9 @@ struct seis3d_session {
     void release() noexcept {
         if (stream) (void)hipStreamSynchronize(stream);
         seis3d::host::release_abc(abc);
-        for (auto& p : fields) { (void)hipFree(p); p = nullptr; }
-        (void)hipFree(inv_rho); (void)hipFree(lambda); (void)hipFree(mu); (void)hipFree(inv_mu);
-        inv_rho = lambda = mu = inv_mu = nullptr;
+        for (auto& p : fields) seis3d::host::free_grid(p);
+        seis3d::host::free_grid(inv_rho); seis3d::host::free_grid(lambda);
+        seis3d::host::free_grid(mu); seis3d::host::free_grid(inv_mu);
         for (int a = 0; a < 3; ++a) { (void)hipFree(psi[a]); (void)hipFree(coef[a]); psi[a] = coef[a] = nullptr; }
         (void)hipFree(flags); (void)hipFree(receivers); (void)hipFree(receiver_values); (void)hipFree(gather);
         flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr;
@@ -162,6 +172,8 @@ namespace seis3d::host {
 
 void require_session(const seis3d_t* t);
 void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what);
+// A zeroed grid array of `cells` floats at kGridLead into its allocation (free with free_grid).
+void alloc_grid(float*& p, std::size_t cells, hipStream_t s, const char* what);
 void check_source(const seis3d_ctx& c, const seis3d_source& s);
 // Source term at time `time` (sourceKernel3D, cuda/Cuda_Seis3D.cu:313-332).
 seis3d::DevSource make_source(const seis3d_ctx& c, const seis3d_source& s, double time);

@@ -208,9 +208,10 @@ def generate(trainer: Trainer, runs: int, size=(48, 48, 48), steps: int = 240, e
 # ---- judging ------------------------------------------------------------------------------------------
 
 def reflection(params, model, src, steps: int, every: int, free_surface: bool = True, frequency: float = 25.0,
-               cases=("learned", "none", "cpml8", "cpml20")) -> dict:
+               cases=("learned", "none", "cpml8", "cpml20"), log=None) -> dict:
     """Relative squared velocity error inside the physical volume against a 40-cell CPML run,
-    summed over every `every`-th step: the reflected energy of each boundary."""
+    summed over every `every`-th step: the reflected energy of each boundary.  `log(message)`,
+    if given, is called after each sampled step."""
     nz, ny, nx = model[0].shape
     vmax = float(model[0].max())
     ref_c = W.imported_context(nx, ny, nz, max_vp=vmax, pml=WIDE_PML, free_surface=free_surface, frequency=frequency)
@@ -240,6 +241,8 @@ def reflection(params, model, src, steps: int, every: int, free_surface: bool = 
                 c, g = runs[case]
                 v = g.download_field(f)[c.oz:c.oz + nz, c.oy:c.oy + ny, c.ox:c.ox + nx]
                 err[case] += float(np.sum((v.astype(np.float64) - r) ** 2))
+        if log:
+            log(f"reflection: step {n + 1}/{steps} " + ", ".join(f"{c} {err[c] / norm:.3e}" for c in cases))
     for c, g in runs.values():
         g.release()
     ref.release()
@@ -301,7 +304,7 @@ def main(argv=None) -> int:
     model = layered_random_model(nx, ny, nz, seed=20261203)
     src = S.default_source(x=nx // 2, y=ny // 2, z=max(2, nz // 4), kind=S.EXPLOSION, amplitude=1e8)
     t0 = time.perf_counter()
-    e = reflection(params, model, src, a.steps, a.every)
+    e = reflection(params, model, src, a.steps, a.every, log=log)
     print(json.dumps({"grid": [nx, ny, nz], "steps": a.steps, "every": a.every, "reflected_energy": e,
                       "gain_vs_none": e["none"] / e["learned"], "gain_vs_cpml8": e["cpml8"] / e["learned"],
                       "seconds": round(time.perf_counter() - t0, 1)}), flush=True)

@@ -8,6 +8,7 @@
 * Training on the GPU lowers the loss.
 * With the trained weights shipped in the package, the learned frame reflects less energy than
   the same frame without its network (truncation); plain 8- and 20-cell CPML are reported beside.
+* Over long runs the dissipative correction keeps the learned frame bounded.
 """
 import numpy as np
 import pytest
@@ -116,3 +117,14 @@ def test_trained_frame_absorbs(gpu):
     e = LA.reflection(params, model, src, steps=360, every=20)
     print("reflected energy (relative squared error vs 40-cell CPML):", e)
     assert e["learned"] < 0.25 * e["none"]
+
+
+def test_learned_frame_is_stable_over_long_runs(gpu):
+    """900 steps at 64^3: the dissipative correction keeps the learned frame bounded (unrestricted
+    one-step corrections grew without bound here) and still well below truncation."""
+    params = LA.load_weights()
+    model = LA.layered_random_model(64, 64, 64, seed=77)
+    src = S.default_source(x=32, y=32, z=16, kind=S.EXPLOSION, amplitude=1e8)
+    e = LA.reflection(params, model, src, steps=900, every=100, cases=("learned", "none"))
+    print("reflected energy over 900 steps:", e)
+    assert np.isfinite(e["learned"]) and e["learned"] < 0.25 * e["none"]
