@@ -246,7 +246,10 @@ void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent
         enqueue_abc(t);
         if (c.free_surface) check(seis3d::launch_velocity_ghost(P, c.spacing, t->stream), "kernel launch");
     }
-    if (c.free_surface) check(seis3d::launch_stress_ghost(P, t->stream), "kernel launch");
+    // The family-4 stress kernel writes the stress ghost planes itself; the learned
+    // frame changes stresses after it, so then the ghosts are redone here.
+    if (c.free_surface && (t->kernels.family != 4 || t->abc))
+        check(seis3d::launch_stress_ghost(P, t->stream), "kernel launch");
     if (c.reflecting) check(seis3d::launch_reflecting_walls(P, false, t->stream), "kernel launch");
     t->ghosts_valid = true;
 }

@@ -395,13 +395,14 @@ constexpr int str_copies() {
 struct StrOperands {
     float o[6];          // stresses at the update plane
     float lam, mu;
-    float r[7];          // 1/mu at i, i+x, i+y, i+z, i+x+y, i+y+z, i+x+z
+    float r[7];          // mu at i+x, i+y, i+z, i+x+y, i+y+z, i+x+z in r[1..6] (r[0] unused: mu)
     float pz[3];         // z-slab CPML memories at the update plane
 };
 
 }  // namespace
 
-// Fused stress half-step (stressKernel3D + explosion source epilogue).
+// Fused stress half-step (stressKernel3D + explosion source epilogue + the free
+// surface's stress ghost planes).
 template <int BY>
 __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     __shared__ StrTiles<BY> ring[3];
@@ -453,9 +454,11 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             for (int c = 0; c < 6; ++c) u.o[c] = P.p.f[3 + c][i];
             u.lam = P.p.lambda[i];
             u.mu = P.p.mu[i];
-            const float* r = P.p.inv_mu;
+            // Shear moduli around the cell: the harmonic means below divide by them
+            // exactly as shearMu3D does (no 1/mu array to read).
+            const float* r = P.p.mu;
             const long long sy = g.px, sz = g.plane;
-            u.r[0] = r[i]; u.r[1] = r[i + 1]; u.r[2] = r[i + sy]; u.r[3] = r[i + sz];
+            u.r[1] = r[i + 1]; u.r[2] = r[i + sy]; u.r[3] = r[i + sz];
             u.r[4] = r[i + 1 + sy]; u.r[5] = r[i + sy + sz]; u.r[6] = r[i + 1 + sz];
             const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
             if (qz >= 0) {
@@ -541,9 +544,9 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float sxx = cur.o[0] + P.dt * (lam * trace + 2 * mu * e00);
             float syy = cur.o[1] + P.dt * (lam * trace + 2 * mu * e11);
             float szz = surface ? 0.0f : cur.o[2] + P.dt * (lam * trace + 2 * mu * d22);
-            const float sxy = cur.o[3] + P.dt * shear_from_inverse(cur.r[0], cur.r[1], cur.r[2], cur.r[4]) * p01[0];
-            const float syz = cur.o[4] + P.dt * shear_from_inverse(cur.r[0], cur.r[2], cur.r[3], cur.r[5]) * (d12 + p21[0]);
-            const float sxz = cur.o[5] + P.dt * shear_from_inverse(cur.r[0], cur.r[1], cur.r[3], cur.r[6]) * (d02 + p20[0]);
+            const float sxy = cur.o[3] + P.dt * shear_harmonic(mu, cur.r[1], cur.r[2], cur.r[4]) * p01[0];
+            const float syz = cur.o[4] + P.dt * shear_harmonic(mu, cur.r[2], cur.r[3], cur.r[5]) * (d12 + p21[0]);
+            const float sxz = cur.o[5] + P.dt * shear_harmonic(mu, cur.r[1], cur.r[3], cur.r[6]) * (d02 + p20[0]);
             if (P.src.active) {
                 const int r2 = source_r2d(P.src, x, y, z);
                 if (r2 >= 0) {
@@ -556,6 +559,19 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
             bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) ||
                    !isfinite(sxz);
+            // Free-surface ghost planes (stressGhostKernel3D, cuda/Cuda_Seis3D.cu:203-219) from
+            // the planes this chunk just updated: normal stresses and sxy even about z = 4,
+            // szz odd about z = 4, sxz and syz odd about z = 3.5.
+            if (g.free_surface && z <= 8) {
+                if (z >= 5) {
+                    const long long gi = dcell(g, x, y, 8 - z);
+                    P.p.f[5][gi] = -szz; P.p.f[3][gi] = sxx; P.p.f[4][gi] = syy; P.p.f[6][gi] = sxy;
+                }
+                if (z <= 7) {
+                    const long long gi = dcell(g, x, y, 7 - z);
+                    P.p.f[8][gi] = -sxz; P.p.f[7][gi] = -syz;
+                }
+            }
         }
         float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
         if (active && k >= z0 && k < z1) {
