@@ -54,6 +54,15 @@ __device__ void load_image(Lds& L, const void* image) {
     for (int k = threadIdx.x; k < kOutPad; k += kBlock) L.b3[k] = b[2 * kHidden + k];
 }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes.  The
+// per-tile phases only touch the wave's own slices of the LDS (xf, xs, h1, h2,
+// scale [wave]), so waves never wait for each other inside the tile loops.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // MLP of one 16-cell tile held in L.xs[wave]; returns lane's layer-3 outputs
 // (rows 4q+j, column lane%16).  Optionally stores BF16-rounded activations.
 __device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long row0, int rows) {
@@ -76,7 +85,7 @@ __device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long ro
             L.h1[wave][4 * q + j][nb * 16 + m] = h;
             if (H1 && 4 * q + j < rows) H1[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
         }
-    __syncthreads();
+    wave_sync();
     floatx4 acc2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
     for (int kb = 0; kb < kHidden / 16; ++kb) {
@@ -95,7 +104,7 @@ __device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long ro
             L.h2[wave][4 * q + j][nb * 16 + m] = h;
             if (H2 && 4 * q + j < rows) H2[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
         }
-    __syncthreads();
+    wave_sync();
     floatx4 out = {0, 0, 0, 0};
 #pragma unroll
     for (int kb = 0; kb < kHidden / 16; ++kb) {
@@ -118,6 +127,23 @@ __device__ __forceinline__ int coord_of(const DevGrid& g, long long i, int axis)
 }
 __device__ __forceinline__ int extent_of(const DevGrid& g, int axis) { return axis == 0 ? g.gx : axis == 1 ? g.gy : g.gz; }
 
+// Field c of a frame.  Selecting by a switch keeps the kernel-argument pointer
+// array in scalar registers; indexing it with a per-lane value would copy it to
+// scratch memory and add a scratch load to every feature load.
+__device__ __forceinline__ const float* field_ptr(const Frame& F, int c) {
+    switch (c) {
+    case 0: return F.f[0];
+    case 1: return F.f[1];
+    case 2: return F.f[2];
+    case 3: return F.f[3];
+    case 4: return F.f[4];
+    case 5: return F.f[5];
+    case 6: return F.f[6];
+    case 7: return F.f[7];
+    default: return F.f[8];
+    }
+}
+
 __device__ __forceinline__ float impedance(const Frame& F, long long i) {
     return sqrtf((F.lambda[i] + 2.0f * F.mu[i]) / F.inv_rho[i]);
 }
@@ -136,7 +162,7 @@ __device__ void gather_row(const Frame& F, int i, unsigned char code, int part, 
         const bool inside = c >= 4 && c < n - 4;
         float v;
         if (ch < 9) {
-            v = F.f[kFaceField[axis][ch]][p];
+            v = field_ptr(F, kFaceField[axis][ch])[p];
             if (!hi && kOdd[ch] != 0.0f) v = -v;
             if (ch >= 3) v = v / z;
         } else if (ch == 9) {
@@ -190,9 +216,9 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
         if (cellr < rows) gather_row(F, C.cell[first + cellr], C.code[first + cellr], part, L.xf[wave][cellr]);
         else
             for (int f = part; f < kIn; f += 4) L.xf[wave][cellr][f] = 0.0f;
-        __syncthreads();
+        wave_sync();
         normalise_tile(L, wave, rows, nullptr, nullptr, 0);
-        __syncthreads();
+        wave_sync();
         const floatx4 out = mlp_tile(L, wave, nullptr, nullptr, 0, 0);
         const int m = lane & 15, q = lane >> 4;
         if (m < kOut) {
@@ -224,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
                 delta[static_cast<long long>(first + r) * kOut + kFaceField[axis][m]] = v;
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -273,7 +299,7 @@ __global__ void face_values_kernel(Frame R, DevGrid tg, FrameCells C, int ox, in
     const long long p = (static_cast<long long>(z + oz) * R.g.gy + (y + oy)) * R.g.px + (x + ox);
     const float zimp = impedance(R, p);
     for (int ch = 0; ch < 9; ++ch) {
-        float v = R.f[kFaceField[axis][ch]][p];
+        float v = field_ptr(R, kFaceField[axis][ch])[p];
         if (!hi && kOdd[ch] != 0.0f) v = -v;
         if (ch >= 3) v = v / zimp;
         out[static_cast<long long>(k) * kOut + ch] = v;
@@ -340,14 +366,14 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const void* image, cons
         const int cellr = lane >> 2, part = lane & 3;
         for (int f = part; f < kIn; f += 4)
             L.xs[wave][cellr][f] = to_bf16(cellr < rows ? X[(first + cellr) * kIn + f] : 0.0f);
-        __syncthreads();
+        wave_sync();
         const floatx4 out = mlp_tile(L, wave, H1, H2, first, rows);
         const int m = lane & 15, q = lane >> 4;
         if (m < kOut)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 if (4 * q + j < rows) Y[(first + 4 * q + j) * kOut + m] = out[j];
-        __syncthreads();
+        wave_sync();
     }
 }
 
