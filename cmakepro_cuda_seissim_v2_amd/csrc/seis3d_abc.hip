@@ -37,10 +37,13 @@ struct Lds {
     unsigned short w2[kHidden * kHidden];
     unsigned short w3[kOutPad * kHidden];
     float b1[kHidden], b2[kHidden], b3[kOutPad];
-    float xf[kWaves][16][kIn];
-    unsigned short xs[kWaves][16][kIn];
-    unsigned short h1[kWaves][16][kHidden];
-    unsigned short h2[kWaves][16][kHidden];
+    // Rows are padded so that the 16 cells of a wave fall in different LDS banks
+    // (96 floats or 96 / 32 BF16 per row would put every row in the same banks);
+    // BF16 rows keep the 8-byte alignment of the MFMA operand loads.
+    float xf[kWaves][16][kIn + 4];
+    unsigned short xs[kWaves][16][kIn + 8];
+    unsigned short h1[kWaves][16][kHidden + 8];
+    unsigned short h2[kWaves][16][kHidden + 8];
     float scale[kWaves][16];
 };
 
@@ -63,25 +66,52 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// MFMA B operands and biases of one lane (column m = lane % 16, k-quad q = lane / 16),
+// held in registers for a kernel's lifetime: reading them from LDS for every tile made
+// most of the waves' time LDS bank conflicts (all 64 lanes of a read hit 8 banks).
+struct RegWeights {
+    bf16x4 w1[kIn / 16][2], w2[kHidden / 16][2], w3[kHidden / 16];
+    float b1[2], b2[2], b3;
+};
+
+__device__ void load_weights(const Lds& L, RegWeights& W) {
+    const int lane = int(threadIdx.x) & 63, m = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int kb = 0; kb < kIn / 16; ++kb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+            W.w1[kb][nb] = *reinterpret_cast<const bf16x4*>(&L.w1[(nb * 16 + m) * kIn + kb * 16 + 4 * q]);
+#pragma unroll
+    for (int kb = 0; kb < kHidden / 16; ++kb) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+            W.w2[kb][nb] = *reinterpret_cast<const bf16x4*>(&L.w2[(nb * 16 + m) * kHidden + kb * 16 + 4 * q]);
+        W.w3[kb] = *reinterpret_cast<const bf16x4*>(&L.w3[m * kHidden + kb * 16 + 4 * q]);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        W.b1[nb] = L.b1[nb * 16 + m];
+        W.b2[nb] = L.b2[nb * 16 + m];
+    }
+    W.b3 = L.b3[m];
+}
+
 // MLP of one 16-cell tile held in L.xs[wave]; returns lane's layer-3 outputs
 // (rows 4q+j, column lane%16).  Optionally stores BF16-rounded activations.
-__device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long row0, int rows) {
+__device__ floatx4 mlp_tile(Lds& L, const RegWeights& W, int wave, float* H1, float* H2, long long row0, int rows) {
     const int lane = int(threadIdx.x) & 63, m = lane & 15, q = lane >> 4;
     floatx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
     for (int kb = 0; kb < kIn / 16; ++kb) {
         const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.xs[wave][m][kb * 16 + 4 * q]);
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w1[(nb * 16 + m) * kIn + kb * 16 + 4 * q]);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
-        }
+        for (int nb = 0; nb < 2; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, W.w1[kb][nb], acc[nb], 0, 0, 0);
     }
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const unsigned short h = to_bf16(fmaxf(acc[nb][j] + L.b1[nb * 16 + m], 0.0f));
+            const unsigned short h = to_bf16(fmaxf(acc[nb][j] + W.b1[nb], 0.0f));
             L.h1[wave][4 * q + j][nb * 16 + m] = h;
             if (H1 && 4 * q + j < rows) H1[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
         }
@@ -91,16 +121,14 @@ __device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long ro
     for (int kb = 0; kb < kHidden / 16; ++kb) {
         const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.h1[wave][m][kb * 16 + 4 * q]);
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w2[(nb * 16 + m) * kHidden + kb * 16 + 4 * q]);
-            acc2[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc2[nb], 0, 0, 0);
-        }
+        for (int nb = 0; nb < 2; ++nb)
+            acc2[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, W.w2[kb][nb], acc2[nb], 0, 0, 0);
     }
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const unsigned short h = to_bf16(fmaxf(acc2[nb][j] + L.b2[nb * 16 + m], 0.0f));
+            const unsigned short h = to_bf16(fmaxf(acc2[nb][j] + W.b2[nb], 0.0f));
             L.h2[wave][4 * q + j][nb * 16 + m] = h;
             if (H2 && 4 * q + j < rows) H2[(row0 + 4 * q + j) * kHidden + nb * 16 + m] = from_bf16(h);
         }
@@ -109,11 +137,10 @@ __device__ floatx4 mlp_tile(Lds& L, int wave, float* H1, float* H2, long long ro
 #pragma unroll
     for (int kb = 0; kb < kHidden / 16; ++kb) {
         const bf16x4 a = *reinterpret_cast<const bf16x4*>(&L.h2[wave][m][kb * 16 + 4 * q]);
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(&L.w3[m * kHidden + kb * 16 + 4 * q]);
-        out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, out, 0, 0, 0);
+        out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, W.w3[kb], out, 0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[j] += L.b3[m];
+    for (int j = 0; j < 4; ++j) out[j] += W.b3;
     return out;
 }
 
@@ -204,6 +231,8 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
     Lds& L = *reinterpret_cast<Lds*>(smem);
     load_image(L, image);
     __syncthreads();
+    RegWeights W;
+    load_weights(L, W);
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
     const int tiles = (C.count + 15) / 16;
     const int waves_total = int(gridDim.x) * kWaves;
@@ -219,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
         wave_sync();
         normalise_tile(L, wave, rows, nullptr, nullptr, 0);
         wave_sync();
-        const floatx4 out = mlp_tile(L, wave, nullptr, nullptr, 0, 0);
+        const floatx4 out = mlp_tile(L, W, wave, nullptr, nullptr, 0, 0);
         const int m = lane & 15, q = lane >> 4;
         if (m < kOut) {
 #pragma unroll
@@ -355,6 +384,8 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const void* image, cons
     Lds& L = *reinterpret_cast<Lds*>(smem);
     load_image(L, image);
     __syncthreads();
+    RegWeights W;
+    load_weights(L, W);
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
     const int tiles = (n + 15) / 16;
     const int waves_total = int(gridDim.x) * kWaves;
@@ -367,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const void* image, cons
         for (int f = part; f < kIn; f += 4)
             L.xs[wave][cellr][f] = to_bf16(cellr < rows ? X[(first + cellr) * kIn + f] : 0.0f);
         wave_sync();
-        const floatx4 out = mlp_tile(L, wave, H1, H2, first, rows);
+        const floatx4 out = mlp_tile(L, W, wave, H1, H2, first, rows);
         const int m = lane & 15, q = lane >> 4;
         if (m < kOut)
 #pragma unroll
