@@ -6,9 +6,12 @@
 // A[row l%16][k 4(l/16)+j] and B[k 4(l/16)+j][col l%16] (j = 0..3) and holds
 // D[row 4(l/16)+j][col l%16].  A wavefront evaluates 16 cells per tile:
 // 6 + 2 MFMAs per 16-column block of the 96->32 and 32->32 layers and 2 for
-// the 32->9 layer.  Weights (9.2 KB BF16) stay in LDS for the whole grid
-// stride loop.
+// the 32->9 layer.  Each lane holds its B operands (9.2 KB BF16 per wave in
+// all) in registers for the whole grid-stride loop.
 #include "seis3d_abc.hpp"
+
+#include <algorithm>
+#include <atomic>
 
 namespace seis3d::abc {
 
@@ -27,39 +30,32 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
 
-// Face-frame component -> global field, per normal axis (normal component first).
-__constant__ int kFaceField[3][9] = {{0, 1, 2, 3, 4, 5, 6, 7, 8}, {1, 2, 0, 4, 5, 3, 7, 8, 6}, {2, 0, 1, 5, 3, 4, 8, 6, 7}};
+// Face-frame component ch -> global field for normal axis `axis` (normal component
+// first): each of the triples (vx, vy, vz), (sxx, syy, szz), (sxy, syz, sxz) rotates
+// cyclically with the axis.
+__host__ __device__ constexpr int face_field(int axis, int ch) { return 3 * (ch / 3) + (ch % 3 + axis) % 3; }
 // Components odd in the normal coordinate (v_n, s_nt1, s_nt2) flip sign on low faces.
-__constant__ float kOdd[9] = {1, 0, 0, 0, 0, 0, 1, 0, 1};
+__host__ __device__ constexpr bool face_odd(int ch) { return ch == 0 || ch == 6 || ch == 8; }
 
+static_assert(face_field(0, 5) == 5 && face_field(1, 0) == 1 && face_field(1, 5) == 3 && face_field(2, 6) == 8 &&
+                  face_field(2, 8) == 7,
+              "face frame rotation");
+
+// Per-wave tile buffers; every phase of a tile touches only its own wave's slices.
 struct Lds {
-    unsigned short w1[kHidden * kIn];
-    unsigned short w2[kHidden * kHidden];
-    unsigned short w3[kOutPad * kHidden];
-    float b1[kHidden], b2[kHidden], b3[kOutPad];
-    // Rows are padded so that the 16 cells of a wave fall in different LDS banks
-    // (96 floats or 96 / 32 BF16 per row would put every row in the same banks);
+    // Rows are padded so that the 16 cells of a wave fall in different LDS banks;
     // BF16 rows keep the 8-byte alignment of the MFMA operand loads.
-    float xf[kWaves][16][kIn + 4];
     unsigned short xs[kWaves][16][kIn + 8];
     unsigned short h1[kWaves][16][kHidden + 8];
     unsigned short h2[kWaves][16][kHidden + 8];
+    float centre[kWaves][16][kOut];   // raw face-frame value of the cell itself (tap 3)
     float scale[kWaves][16];
+    float imp[kWaves][16];            // impedance at the cell
 };
 
-__device__ void load_image(Lds& L, const void* image) {
-    const unsigned short* w = static_cast<const unsigned short*>(image);
-    for (int k = threadIdx.x; k < kHidden * kIn; k += kBlock) L.w1[k] = w[k];
-    for (int k = threadIdx.x; k < kHidden * kHidden; k += kBlock) L.w2[k] = w[kHidden * kIn + k];
-    for (int k = threadIdx.x; k < kOutPad * kHidden; k += kBlock) L.w3[k] = w[kHidden * kIn + kHidden * kHidden + k];
-    const float* b = reinterpret_cast<const float*>(w + kBf16Count);
-    for (int k = threadIdx.x; k < kHidden; k += kBlock) { L.b1[k] = b[k]; L.b2[k] = b[kHidden + k]; }
-    for (int k = threadIdx.x; k < kOutPad; k += kBlock) L.b3[k] = b[2 * kHidden + k];
-}
-
 // Orders this wave's LDS writes before its later LDS reads by other lanes.  The
-// per-tile phases only touch the wave's own slices of the LDS (xf, xs, h1, h2,
-// scale [wave]), so waves never wait for each other inside the tile loops.
+// per-tile phases only touch the wave's own slices of the LDS, so waves never wait
+// for each other inside the tile loops.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -67,33 +63,39 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // MFMA B operands and biases of one lane (column m = lane % 16, k-quad q = lane / 16),
-// held in registers for a kernel's lifetime: reading them from LDS for every tile made
-// most of the waves' time LDS bank conflicts (all 64 lanes of a read hit 8 banks).
+// read once from the weight image and held in registers for a kernel's lifetime.
 struct RegWeights {
     bf16x4 w1[kIn / 16][2], w2[kHidden / 16][2], w3[kHidden / 16];
     float b1[2], b2[2], b3;
 };
 
-__device__ void load_weights(const Lds& L, RegWeights& W) {
+// Image layout: BF16 W1 [kHidden][kIn], W2 [kHidden][kHidden], W3 [kOutPad][kHidden],
+// then FP32 b1 [kHidden], b2 [kHidden], b3 [kOutPad].  Every operand offset is a multiple
+// of four BF16 values, so the 8-byte loads are aligned.
+__device__ void load_weights(const void* image, RegWeights& W) {
+    const unsigned short* w = static_cast<const unsigned short*>(image);
+    const unsigned short* w2 = w + kHidden * kIn;
+    const unsigned short* w3 = w2 + kHidden * kHidden;
+    const float* b = reinterpret_cast<const float*>(w + kBf16Count);
     const int lane = int(threadIdx.x) & 63, m = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int kb = 0; kb < kIn / 16; ++kb)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
-            W.w1[kb][nb] = *reinterpret_cast<const bf16x4*>(&L.w1[(nb * 16 + m) * kIn + kb * 16 + 4 * q]);
+            W.w1[kb][nb] = *reinterpret_cast<const bf16x4*>(&w[(nb * 16 + m) * kIn + kb * 16 + 4 * q]);
 #pragma unroll
     for (int kb = 0; kb < kHidden / 16; ++kb) {
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
-            W.w2[kb][nb] = *reinterpret_cast<const bf16x4*>(&L.w2[(nb * 16 + m) * kHidden + kb * 16 + 4 * q]);
-        W.w3[kb] = *reinterpret_cast<const bf16x4*>(&L.w3[m * kHidden + kb * 16 + 4 * q]);
+            W.w2[kb][nb] = *reinterpret_cast<const bf16x4*>(&w2[(nb * 16 + m) * kHidden + kb * 16 + 4 * q]);
+        W.w3[kb] = *reinterpret_cast<const bf16x4*>(&w3[m * kHidden + kb * 16 + 4 * q]);
     }
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-        W.b1[nb] = L.b1[nb * 16 + m];
-        W.b2[nb] = L.b2[nb * 16 + m];
+        W.b1[nb] = b[nb * 16 + m];
+        W.b2[nb] = b[kHidden + nb * 16 + m];
     }
-    W.b3 = L.b3[m];
+    W.b3 = b[2 * kHidden + m];
 }
 
 // MLP of one 16-cell tile held in L.xs[wave]; returns lane's layer-3 outputs
@@ -175,65 +177,75 @@ __device__ __forceinline__ float impedance(const Frame& F, long long i) {
     return sqrtf((F.lambda[i] + 2.0f * F.mu[i]) / F.inv_rho[i]);
 }
 
-// The 96 raw (un-normalised) features of feature indices part, part+4, ... of cell i into xf row.
-__device__ void gather_row(const Frame& F, int i, unsigned char code, int part, float* row) {
-    const int axis = code >> 1;
-    const bool hi = (code & 1) != 0;
-    const long long st = stride_of(F.g, axis) * (hi ? 1 : -1);
-    const int c0 = coord_of(F.g, i, axis), n = extent_of(F.g, axis);
+// Features per lane: the 96 features (12 channels x 8 taps) of a cell are split over 4 lanes.
+constexpr int kPerLane = kIn / 4;
+
+// Raw (un-normalised) features part + 4k (k = 0..23) of frame cell i on a face normal to
+// AXIS; returns the impedance at the cell.  Feature f is tap f % 8 of channel f / 8 (tap 3
+// = the cell, taps run outwards along the normal).  With part < 4, feature part + 4k is
+// channel k / 2, tap 4 (k & 1) + part, so once the loop is unrolled every channel's field,
+// sign and scaling are compile-time constants, a lane's loads use two cell offsets, and
+// all of them issue before the first result is used.
+template <int AXIS>
+__device__ __forceinline__ float gather_axis(const Frame& F, int i, bool hi, int part, float (&feat)[kPerLane]) {
+    const long long st = stride_of(F.g, AXIS) * (hi ? 1 : -1);
+    const int c0 = coord_of(F.g, i, AXIS), n = extent_of(F.g, AXIS);
     const float z = impedance(F, i);
-    for (int f = part; f < kIn; f += 4) {
-        const int ch = f / kTaps, tap = f % kTaps, o = tap - 3;
+    long long p[2];
+    bool inside[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int o = 4 * h + part - 3;
         const int c = c0 + (hi ? o : -o);
-        const long long p = i + o * st;
-        const bool inside = c >= 4 && c < n - 4;
-        float v;
-        if (ch < 9) {
-            v = field_ptr(F, kFaceField[axis][ch])[p];
-            if (!hi && kOdd[ch] != 0.0f) v = -v;
-            if (ch >= 3) v = v / z;
-        } else if (ch == 9) {
-            v = inside ? 1.0f : 0.0f;
-        } else if (ch == 10) {
-            v = sqrtf((F.lambda[p] + 2.0f * F.mu[p]) * F.inv_rho[p]) * (1.0f / 5000.0f);
-        } else {
-            v = inside ? (hi ? 1.0f : -1.0f) : 0.0f;
-        }
-        row[f] = v;
+        p[h] = i + o * st;
+        inside[h] = c >= 4 && c < n - 4;
     }
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+        const int ch = k / 2, h = k & 1;
+        float x;
+        if (ch < 9) {
+            x = F.f[face_field(AXIS, ch)][p[h]];
+            if (!hi && face_odd(ch)) x = -x;
+            if (ch >= 3) x = x / z;
+        } else if (ch == 9) {
+            x = inside[h] ? 1.0f : 0.0f;
+        } else if (ch == 10) {
+            x = sqrtf((F.lambda[p[h]] + 2.0f * F.mu[p[h]]) * F.inv_rho[p[h]]) * (1.0f / 5000.0f);
+        } else {
+            x = inside[h] ? (hi ? 1.0f : -1.0f) : 0.0f;
+        }
+        feat[k] = x;
+    }
+    return z;
 }
 
-// Scale s = max |field feature| of each tile row; rows normalised into xs (BF16) / X (FP32).
-// A window quieter than kMinScale is silent: zero features and zero scale, so no
-// correction (1 / s would overflow for denormal s and turn zero taps into NaN).
-__device__ void normalise_tile(Lds& L, int wave, int rows, float* X, float* S, long long row0) {
-    const int lane = int(threadIdx.x) & 63, cellr = lane >> 2, part = lane & 3;
-    float mx = 0.0f;
-    for (int f = part; f < 9 * kTaps; f += 4) mx = fmaxf(mx, fabsf(L.xf[wave][cellr][f]));
-    mx = fmaxf(mx, __shfl_xor(mx, 1));
-    mx = fmaxf(mx, __shfl_xor(mx, 2));
-    const bool live = mx >= kMinScale;
-    const float inv = live ? 1.0f / mx : 0.0f;
-    for (int f = part; f < kIn; f += 4) {
-        const float v = f < 9 * kTaps ? L.xf[wave][cellr][f] * inv : (live ? L.xf[wave][cellr][f] : 0.0f);
-        L.xs[wave][cellr][f] = to_bf16(v);
-        if (X && cellr < rows) X[(row0 + cellr) * kIn + f] = v;
-    }
-    if (part == 0) {
-        L.scale[wave][cellr] = live ? mx : 0.0f;
-        if (S && cellr < rows) S[row0 + cellr] = live ? mx : 0.0f;
+// Frame cells are listed in grid order, so the 16 cells of a tile nearly always share
+// a normal axis and a wave takes one branch.  A constant axis keeps every field
+// pointer a kernel-argument load (a lane-varying index into the pointer array would
+// copy the whole argument block to scratch memory).
+__device__ __forceinline__ float gather_features(const Frame& F, int i, unsigned char code, int part,
+                                                 float (&feat)[kPerLane]) {
+    const bool hi = (code & 1) != 0;
+    switch (code >> 1) {
+    case 0: return gather_axis<0>(F, i, hi, part, feat);
+    case 1: return gather_axis<1>(F, i, hi, part, feat);
+    default: return gather_axis<2>(F, i, hi, part, feat);
     }
 }
 
 // Inference: corrections of all frame cells into delta [count][9] (global field order).
-__global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, const void* image, float* delta) {
+// Lane (cell r = lane / 4, part = lane % 4) gathers and normalises its 24 features in
+// registers; LDS holds only the BF16 tile, the activations and three values per cell.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void correct_kernel(Frame F, FrameCells C,
+                                                                                                 const void* image,
+                                                                                                 float* delta) {
     extern __shared__ unsigned char smem[];
     Lds& L = *reinterpret_cast<Lds*>(smem);
-    load_image(L, image);
-    __syncthreads();
     RegWeights W;
-    load_weights(L, W);
+    load_weights(image, W);
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const int cellr = lane >> 2, part = lane & 3;
     const int tiles = (C.count + 15) / 16;
     const int waves_total = int(gridDim.x) * kWaves;
     const int iters = (tiles + waves_total - 1) / waves_total;
@@ -241,12 +253,36 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
         const int tile = it * waves_total + int(blockIdx.x) * kWaves + wave;
         const int first = tile * 16;
         const int rows = tile < tiles ? min(16, C.count - first) : 0;
-        const int cellr = lane >> 2, part = lane & 3;
-        if (cellr < rows) gather_row(F, C.cell[first + cellr], C.code[first + cellr], part, L.xf[wave][cellr]);
-        else
-            for (int f = part; f < kIn; f += 4) L.xf[wave][cellr][f] = 0.0f;
-        wave_sync();
-        normalise_tile(L, wave, rows, nullptr, nullptr, 0);
+        float feat[kPerLane];
+        float z = 0.0f;
+        if (cellr < rows) {
+            z = gather_features(F, C.cell[first + cellr], C.code[first + cellr], part, feat);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPerLane; ++k) feat[k] = 0.0f;
+        }
+        // Scale s = max |field feature| of the cell (its first 72 features, k < 18 on each
+        // of its 4 lanes).  A window quieter than kMinScale is silent: zero features and
+        // zero scale, so no correction (1 / s would overflow for denormal s and turn zero
+        // taps into NaN).
+        float mx = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) mx = fmaxf(mx, fabsf(feat[k]));
+        mx = fmaxf(mx, __shfl_xor(mx, 1));
+        mx = fmaxf(mx, __shfl_xor(mx, 2));
+        const bool live = mx >= kMinScale;
+        const float inv = live ? 1.0f / mx : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k)
+            L.xs[wave][cellr][part + 4 * k] = to_bf16(k < 18 ? feat[k] * inv : (live ? feat[k] : 0.0f));
+        if (part == 3) {
+#pragma unroll
+            for (int ch = 0; ch < kOut; ++ch) L.centre[wave][cellr][ch] = feat[2 * ch];   // tap 3 of channel ch
+        }
+        if (part == 0) {
+            L.scale[wave][cellr] = live ? mx : 0.0f;
+            L.imp[wave][cellr] = z;
+        }
         wave_sync();
         const floatx4 out = mlp_tile(L, W, wave, nullptr, nullptr, 0, 0);
         const int m = lane & 15, q = lane >> 4;
@@ -255,7 +291,6 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
             for (int j = 0; j < 4; ++j) {
                 const int r = 4 * q + j;
                 if (r >= rows) continue;
-                const int i = C.cell[first + r];
                 const unsigned char code = C.code[first + r];
                 const int axis = code >> 1;
                 const bool hi = (code & 1) != 0;
@@ -266,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
                 const float sc = L.scale[wave][r];
                 float d = out[j];
                 if (sc > 0.0f) {
-                    const float x = L.xf[wave][r][m * kTaps + 3] / sc;
+                    const float x = L.centre[wave][r][m] / sc;
                     const float mag = fabsf(x);
                     const float kept = fminf(fmaxf(copysignf(1.0f, x) * (x + d), 0.0f), mag);
                     d = copysignf(kept, x) - x;
@@ -274,9 +309,9 @@ __global__ __launch_bounds__(kBlock) void correct_kernel(Frame F, FrameCells C, 
                     d = 0.0f;
                 }
                 float v = d * sc;
-                if (m >= 3) v *= impedance(F, i);
-                if (!hi && kOdd[m] != 0.0f) v = -v;
-                delta[static_cast<long long>(first + r) * kOut + kFaceField[axis][m]] = v;
+                if (m >= 3) v *= L.imp[wave][r];
+                if (!hi && face_odd(m)) v = -v;
+                delta[static_cast<long long>(first + r) * kOut + face_field(axis, m)] = v;
             }
         }
         wave_sync();
@@ -301,7 +336,13 @@ __global__ __launch_bounds__(128) void features_kernel(Frame F, FrameCells C, fl
     float row[kIn];
     const int i = C.cell[k];
     const unsigned char code = C.code[k];
-    for (int p = 0; p < 4; ++p) gather_row(F, i, code, p, row);
+#pragma unroll
+    for (int part = 0; part < 4; ++part) {
+        float feat[kPerLane];
+        gather_features(F, i, code, part, feat);
+#pragma unroll
+        for (int j = 0; j < kPerLane; ++j) row[part + 4 * j] = feat[j];
+    }
     float mx = 0.0f;
     for (int f = 0; f < 9 * kTaps; ++f) mx = fmaxf(mx, fabsf(row[f]));
     const bool live = mx >= kMinScale;
@@ -328,8 +369,8 @@ __global__ void face_values_kernel(Frame R, DevGrid tg, FrameCells C, int ox, in
     const long long p = (static_cast<long long>(z + oz) * R.g.gy + (y + oy)) * R.g.px + (x + ox);
     const float zimp = impedance(R, p);
     for (int ch = 0; ch < 9; ++ch) {
-        float v = field_ptr(R, kFaceField[axis][ch])[p];
-        if (!hi && kOdd[ch] != 0.0f) v = -v;
+        float v = field_ptr(R, face_field(axis, ch))[p];
+        if (!hi && face_odd(ch)) v = -v;
         if (ch >= 3) v = v / zimp;
         out[static_cast<long long>(k) * kOut + ch] = v;
     }
@@ -382,10 +423,8 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const void* image, cons
                                                          float* Y) {
     extern __shared__ unsigned char smem[];
     Lds& L = *reinterpret_cast<Lds*>(smem);
-    load_image(L, image);
-    __syncthreads();
     RegWeights W;
-    load_weights(L, W);
+    load_weights(image, W);
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
     const int tiles = (n + 15) / 16;
     const int waves_total = int(gridDim.x) * kWaves;
@@ -544,10 +583,36 @@ constexpr int kSumBlocks = 256;
 
 }  // namespace
 
+// Blocks of correct_kernel resident on the whole device at once, cached per device: the
+// grid-stride loop then runs every block for the same number of tiles, with no partial
+// last round of blocks.
+hipError_t resident_correct_blocks(int* blocks) {
+    constexpr int kMaxDevices = 64;
+    static std::atomic<int> cache[kMaxDevices];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const int cached = dev >= 0 && dev < kMaxDevices ? cache[dev].load(std::memory_order_relaxed) : 0;
+    if (cached > 0) {
+        *blocks = cached;
+        return hipSuccess;
+    }
+    int per_cu = 0, cus = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, correct_kernel, kBlock, sizeof(Lds));
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    *blocks = std::max(1, per_cu) * std::max(1, cus);
+    if (dev >= 0 && dev < kMaxDevices) cache[dev].store(*blocks, std::memory_order_relaxed);
+    return hipSuccess;
+}
+
 hipError_t launch_correct(const Frame& F, const FrameCells& cells, const void* image, float* delta, hipStream_t s) {
     if (cells.count <= 0) return hipSuccess;
+    int resident = 0;
+    const hipError_t e = resident_correct_blocks(&resident);
+    if (e != hipSuccess) return e;
     const int tiles = (cells.count + 15) / 16;
-    const unsigned blocks = unsigned(std::min(2048, (tiles + kWaves - 1) / kWaves));
+    const unsigned blocks = unsigned(std::min(resident, (tiles + kWaves - 1) / kWaves));
     hipLaunchKernelGGL(correct_kernel, dim3(blocks), dim3(kBlock), sizeof(Lds), s, F, cells, image, delta);
     return hipGetLastError();
 }
