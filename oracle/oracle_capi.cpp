@@ -191,6 +191,52 @@ int oracle3d_fwi_gradient(Handle* h, const seis3d_source* src, int64_t n_steps, 
     });
 }
 
+// CPML memories of one axis: every slot, in the solver's slab layout.
+uint64_t oracle3d_cpml_size(Handle* h, int axis) {
+    return with_solver(h, [&](auto& s) { return uint64_t(s.cpml[axis].size()); });
+}
+
+int oracle3d_get_cpml(Handle* h, int axis, double* out) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            const auto& v = s.cpml[axis];
+            for (std::size_t i = 0; i < v.size(); ++i) out[i] = double(v[i]);
+        });
+    });
+}
+
+int oracle3d_set_cpml(Handle* h, int axis, const double* in) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            using Real = std::decay_t<decltype(s.f[0][0])>;
+            auto& v = s.cpml[axis];
+            for (std::size_t i = 0; i < v.size(); ++i) v[i] = Real(in[i]);
+        });
+    });
+}
+
+// Adjoint of one step n applied in place to an adjoint state laid out as the nine
+// wavefields over the padded grid followed by the three CPML memory arrays.  The
+// solver's own state stands in for the stored forward states, which only enter the
+// model-gradient sums (discarded here).  All-CPML contexts only.
+int oracle3d_adjoint_step(Handle* h, const seis3d_source* src, uint64_t step, double* state) {
+    return guard([&] {
+        with_solver(h, [&](auto& s) {
+            if (s.c.free_surface || s.c.reflecting)
+                throw std::invalid_argument("Seis3D: the adjoint supports all-CPML boundaries only.");
+            auto A = s.make_adjoint();
+            double* p = state;
+            for (auto& v : A.f) for (auto& x : v) x = *p++;
+            for (auto& v : A.cpml) for (auto& x : v) x = *p++;
+            const auto snap = s.save();
+            s.adjoint_step(snap, snap, A, *src, step);
+            p = state;
+            for (const auto& v : A.f) for (const auto x : v) *p++ = double(x);
+            for (const auto& v : A.cpml) for (const auto x : v) *p++ = double(x);
+        });
+    });
+}
+
 int oracle3d_nonfinite(Handle* h) {
     return with_solver(h, [](auto& s) { return s.nonfinite ? 1 : 0; });
 }

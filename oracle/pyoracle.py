@@ -59,6 +59,10 @@ def library() -> C.CDLL:
             "oracle3d_set_field": (C.c_int, [vp, C.c_int, P(C.c_double)]),
             "oracle3d_sample_scalar": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
             "oracle3d_nonfinite": (C.c_int, [vp]),
+            "oracle3d_cpml_size": (C.c_uint64, [vp, C.c_int]),
+            "oracle3d_get_cpml": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+            "oracle3d_set_cpml": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+            "oracle3d_adjoint_step": (C.c_int, [vp, P(Source), C.c_uint64, P(C.c_double)]),
             "oracle3d_fwi_gradient": (C.c_int, [vp, P(Source), C.c_int64, C.c_int, P(C.c_double), P(C.c_double),
                                                 P(C.c_double), P(C.c_double), P(C.c_double)]),
         }
@@ -204,6 +208,38 @@ class Oracle3D:
     def set_field(self, f: int, values: np.ndarray) -> None:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         _check(library().oracle3d_set_field(self._h, int(f), _dptr(a)))
+
+    def cpml(self, axis: int) -> np.ndarray:
+        """CPML memories of one axis (all slots, slab layout)."""
+        out = np.zeros(int(library().oracle3d_cpml_size(self._h, int(axis))))
+        _check(library().oracle3d_get_cpml(self._h, int(axis), _dptr(out)))
+        return out
+
+    def set_cpml(self, axis: int, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != int(library().oracle3d_cpml_size(self._h, int(axis))):
+            raise ValueError("CPML memory size mismatch")
+        _check(library().oracle3d_set_cpml(self._h, int(axis), _dptr(a)))
+
+    def adjoint_step(self, src: Source, step: int, fields, cpml):
+        """Adjoint of step `step` applied to an adjoint state (9 padded wavefields, 3 CPML arrays).
+
+        Returns the new (fields, cpml); the model-gradient sums are discarded.
+        """
+        sizes = [int(library().oracle3d_cpml_size(self._h, a)) for a in range(3)]
+        flat = np.concatenate([np.asarray(f, np.float64).reshape(-1) for f in fields] +
+                              [np.asarray(c, np.float64).reshape(-1) for c in cpml])
+        cells = self.ctx.gx * self.ctx.gy * self.ctx.gz
+        if flat.size != 9 * cells + sum(sizes):
+            raise ValueError("adjoint state size mismatch")
+        _check(library().oracle3d_adjoint_step(self._h, C.byref(src), int(step), _dptr(flat)))
+        shape = (self.ctx.gz, self.ctx.gy, self.ctx.gx)
+        out_f = [flat[k * cells:(k + 1) * cells].reshape(shape).copy() for k in range(9)]
+        o, out_c = 9 * cells, []
+        for n in sizes:
+            out_c.append(flat[o:o + n].copy())
+            o += n
+        return out_f, out_c
 
     def sample_scalar(self, x: int, y: int, z: int, component: int) -> float:
         out = C.c_double()

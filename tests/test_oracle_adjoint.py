@@ -1,5 +1,12 @@
 """Pins the oracle's discrete adjoint (FWI gradient w.r.t. vp, vs, rho) in FP64.
 
+Two independent checks. First, the adjointness (dot-product) test of one step: for a fixed
+model and a zero source the step S is linear in the state (velocities, stresses, CPML
+memories), and the adjoint recursion must satisfy <S x, y> = <x, S^T y> to rounding for random
+x, y. It is run block by block (x on one part of the state, y on one part), so that no
+identity term can hide an error in a small off-diagonal block. Second, the model gradient
+against central differences:
+
 The adjoint has no reference counterpart; what pins it is calculus: for a misfit
 J(m) = 1/2 sum (gather(m) - observed)^2 the adjoint gradient g must satisfy
 <g, step> = (J(m + step) - J(m - step)) / 2 + O(|step|^3) along any direction, so the
@@ -96,6 +103,99 @@ def test_mismatch_is_second_order(request, name, param, seed):
         fd, predicted = directional(case, param, seed, scale)
         errs.append(abs(fd - predicted) / abs(fd))
     assert 5.0 < errs[0] / errs[1] < 15.0, errs
+
+
+# State blocks of the dot-product test: velocities, stresses, and the CPML memories of the
+# stress derivatives (slots 0-2, used by the velocity half-step) and of the velocity
+# derivatives (slots 3-5, used by the stress half-step).
+BLOCKS = ("v", "sigma", "psi_sigma", "psi_v")
+
+
+def random_state(rng, ctx, sizes, block):
+    """Random values on one block, zero elsewhere; wavefields zero in the ghost cells."""
+    fields = [np.zeros((ctx.gz, ctx.gy, ctx.gx)) for _ in range(9)]
+    cpml = [np.zeros(n) for n in sizes]
+    inner = (slice(4, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
+    if block in ("v", "sigma"):
+        for k in (range(3) if block == "v" else range(3, 9)):
+            fields[k][inner] = rng.standard_normal(fields[k][inner].shape)
+    else:
+        first = 0 if block == "psi_sigma" else 3
+        for a, n in enumerate(sizes):
+            slab = n // 6
+            cpml[a][first * slab:(first + 3) * slab] = rng.standard_normal(3 * slab)
+    return fields, cpml
+
+
+def restrict(ctx, sizes, fields, cpml, block):
+    """The part of a state on one block, flattened."""
+    if block in ("v", "sigma"):
+        ks = range(3) if block == "v" else range(3, 9)
+        return np.concatenate([fields[k].reshape(-1) for k in ks])
+    first = 0 if block == "psi_sigma" else 3
+    return np.concatenate([c[first * (n // 6):(first + 3) * (n // 6)] for c, n in zip(cpml, sizes)])
+
+
+@pytest.fixture(scope="module")
+def linear_step():
+    """An FP64 oracle on a heterogeneous all-CPML model and a zero-amplitude source."""
+    nx = ny = nz = 16
+    start, _ = F.models(nx, ny, nz)
+    ctx = F.context(nx, ny, nz, start)
+    src = S.default_source(x=8, y=8, z=8, kind=S.FORCE, amplitude=0.0, direction=(0.6, 0.0, 0.8))
+    o = O.Oracle3D(ctx, 64)
+    o.upload_model(*start)
+    yield o, ctx, src
+    o.close()
+
+
+@pytest.mark.parametrize("block_in", BLOCKS)
+@pytest.mark.parametrize("block_out", BLOCKS)
+def test_adjoint_step_is_the_transpose_of_the_step(linear_step, block_in, block_out):
+    o, ctx, src = linear_step
+    sizes = [o.cpml(a).size for a in range(3)]
+    rng = np.random.default_rng(1000 + 10 * BLOCKS.index(block_in) + BLOCKS.index(block_out))
+    x_f, x_c = random_state(rng, ctx, sizes, block_in)
+    y_f, y_c = random_state(rng, ctx, sizes, block_out)
+    for k in range(9):
+        o.set_field(k, x_f[k])
+    for a in range(3):
+        o.set_cpml(a, x_c[a])
+    step = 37
+    o.step(src, step)
+    sx_f, sx_c = [o.field(k) for k in range(9)], [o.cpml(a) for a in range(3)]
+    sty_f, sty_c = o.adjoint_step(src, step, y_f, y_c)
+    sx = restrict(ctx, sizes, sx_f, sx_c, block_out)
+    sty = restrict(ctx, sizes, sty_f, sty_c, block_in)
+    lhs = float(np.dot(sx, restrict(ctx, sizes, y_f, y_c, block_out)))
+    rhs = float(np.dot(restrict(ctx, sizes, x_f, x_c, block_in), sty))
+    scale = max(np.linalg.norm(sx) * np.linalg.norm(restrict(ctx, sizes, y_f, y_c, block_out)),
+                np.linalg.norm(sty) * np.linalg.norm(restrict(ctx, sizes, x_f, x_c, block_in)))
+    assert abs(lhs - rhs) <= 1e-12 * scale, (block_in, block_out, lhs, rhs, scale)
+
+
+def test_adjointness_detects_a_planted_error(linear_step):
+    """The dot-product test fails when the adjoint state is perturbed by a relative 1e-9 on one
+    block: its tolerance is far below any plausible mistake in the recursion."""
+    o, ctx, src = linear_step
+    sizes = [o.cpml(a).size for a in range(3)]
+    rng = np.random.default_rng(7)
+    x_f, x_c = random_state(rng, ctx, sizes, "v")
+    y_f, y_c = random_state(rng, ctx, sizes, "sigma")
+    for k in range(9):
+        o.set_field(k, x_f[k])
+    for a in range(3):
+        o.set_cpml(a, x_c[a])
+    o.step(src, 3)
+    sx = restrict(ctx, sizes, [o.field(k) for k in range(9)], [o.cpml(a) for a in range(3)], "sigma")
+    sty_f, sty_c = o.adjoint_step(src, 3, y_f, y_c)
+    sty = restrict(ctx, sizes, sty_f, sty_c, "v")
+    planted = sty * (1 + 1e-9 * rng.standard_normal(sty.shape))
+    lhs = float(np.dot(sx, restrict(ctx, sizes, y_f, y_c, "sigma")))
+    xv = restrict(ctx, sizes, x_f, x_c, "v")
+    scale = np.linalg.norm(sty) * np.linalg.norm(xv)
+    assert abs(lhs - float(np.dot(xv, sty))) <= 1e-12 * scale
+    assert abs(lhs - float(np.dot(xv, planted))) > 1e-12 * scale
 
 
 def test_gradient_rejects_free_surface():
