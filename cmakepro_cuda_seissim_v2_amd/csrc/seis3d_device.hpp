@@ -32,15 +32,30 @@ struct DevGrid {
 };
 
 struct DevFields {
-    float* f[9];          // vx vy vz sxx syy szz sxy syz sxz
+    float* f[9];          // vx vy vz sxx syy szz sxy syz sxz (read)
     float* inv_rho;
     float* lambda;
     float* mu;
     float* inv_mu;        // 1/mu per cell, rebuilt after every material change
-    float* psi[3];
+    float* psi[3];        // CPML memories (read)
     const float* coef[3];
     unsigned int* flags;  // [0] non-finite flag
+    // Where a half-step writes the fields it updates and its CPML memories.  Equal to
+    // f / psi for the usual in-place step; a half-step reads its own cell's old value
+    // before writing it, so distinct arrays give an out-of-place step with the same
+    // arithmetic.  Only the LDS-DMA kernels (family 4) take distinct arrays.
+    float* out[9];
+    float* psi_out[3];
 };
+
+// True when a step writes somewhere other than its inputs (an out-of-place step).
+inline bool out_of_place(const DevFields& p) {
+    for (int k = 0; k < 9; ++k)
+        if (p.out[k] != p.f[k]) return true;
+    for (int a = 0; a < 3; ++a)
+        if (p.psi_out[a] != p.psi[a]) return true;
+    return false;
+}
 
 // One source term of sourceKernel3D (cuda/Cuda_Seis3D.cu:313-332); the
 // time function is evaluated on the host in FP64 and rounded to float.

@@ -90,6 +90,18 @@ __device__ __forceinline__ void wait_for_ring(bool next_full) {
     else wait_vm<LITE>();
 }
 
+// Arrays a half-step writes: its inputs for the in-place step, the output arrays for
+// an out-of-place step.  The choice is a template parameter, so the in-place kernels
+// carry no second set of pointers (and no extra registers for them).
+template <bool OOP>
+__device__ __forceinline__ float* out_field(const DevStep& P, int c) {
+    return OOP ? P.p.out[c] : P.p.f[c];
+}
+template <bool OOP>
+__device__ __forceinline__ float* out_psi(const DevStep& P, int a) {
+    return OOP ? P.p.psi_out[a] : P.p.psi[a];
+}
+
 template <int BY>
 struct VelTiles {
     float xx[BY * 72];
@@ -140,7 +152,7 @@ __device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, in
 }  // namespace
 
 // Fused velocity half-step (velocityKernel3D + force source epilogue).
-template <int BY>
+template <int BY, bool OOP>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma(DevStep P, TileMap M) {
     __shared__ VelTiles<BY> ring[3];
     __shared__ float cx[6 * 64];          // x CPML table: [half*3 + k][lane]
@@ -261,7 +273,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             float dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[0] + c0[0] * dz;
-                P.p.psi[2][psi_index_z(g, 0, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 0, qz, x, y)] = m;
                 dz = c0[2] * dz + m;
             }
             const float div_x = p_vx[0] + dz;
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[1] + c0[0] * dz;
-                P.p.psi[2][psi_index_z(g, 1, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 1, qz, x, y)] = m;
                 dz = c0[2] * dz + m;
             }
             const float div_y = p_vy[0] + dz;
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[2] + c1[0] * dz;
-                P.p.psi[2][psi_index_z(g, 2, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 2, qz, x, y)] = m;
                 dz = c1[2] * dz + m;
             }
             const float div_z = p_vz[0] + dz;
@@ -296,7 +308,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
                 }
             }
             const long long i = dcell(g, x, y, z);
-            P.p.f[0][i] = vx; P.p.f[1][i] = vy; P.p.f[2][i] = vz;
+            out_field<OOP>(P, 0)[i] = vx; out_field<OOP>(P, 1)[i] = vy; out_field<OOP>(P, 2)[i] = vz;
             bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
         }
         float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
@@ -308,7 +320,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             float d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[0] + cx[3 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 0, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 0, qx, y, k)] = m;
                 d = cx[5 * 64 + tx] * d + m;
             }
             pvx += d;
@@ -317,7 +329,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[0] + cyt[0 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 0, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 0, qy, x, k)] = m;
                 d = cyt[2 * BY + ty] * d + m;
             }
             pvx += d;
@@ -327,7 +339,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[1] + cx[0 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 1, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 1, qx, y, k)] = m;
                 d = cx[2 * 64 + tx] * d + m;
             }
             pvy += d;
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[1] + cyt[3 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 1, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 1, qy, x, k)] = m;
                 d = cyt[5 * BY + ty] * d + m;
             }
             pvy += d;
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[2] + cx[0 * 64 + tx] * d;
-                P.p.psi[0][psi_index_x(g, 2, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 2, qx, y, k)] = m;
                 d = cx[2 * 64 + tx] * d + m;
             }
             pvz += d;
@@ -355,7 +367,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[2] + cyt[0 * BY + ty] * d;
-                P.p.psi[1][psi_index_y(g, 2, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 2, qy, x, k)] = m;
                 d = cyt[2 * BY + ty] * d + m;
             }
             pvz += d;
@@ -403,7 +415,7 @@ struct StrOperands {
 
 // Fused stress half-step (stressKernel3D + explosion source epilogue + the free
 // surface's stress ghost planes).
-template <int BY>
+template <int BY, bool OOP>
 __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     __shared__ StrTiles<BY> ring[3];
     __shared__ float cx[6 * 64];
@@ -517,7 +529,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float d02 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[0] + c1[0] * d02;
-                P.p.psi[2][psi_index_z(g, 3, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 3, qz, x, y)] = m;
                 d02 = c1[2] * d02 + m;
             }
 #pragma unroll
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float d12 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[1] + c1[0] * d12;
-                P.p.psi[2][psi_index_z(g, 4, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 4, qz, x, y)] = m;
                 d12 = c1[2] * d12 + m;
             }
 #pragma unroll
@@ -533,7 +545,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float d22 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[2] + c0[0] * d22;
-                P.p.psi[2][psi_index_z(g, 5, qz, x, y)] = m;
+                out_psi<OOP>(P, 2)[psi_index_z(g, 5, qz, x, y)] = m;
                 d22 = c0[2] * d22 + m;
             }
             const bool surface = g.free_surface && z == 4;
@@ -555,8 +567,8 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
                 }
             }
             const long long i = dcell(g, x, y, z);
-            P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
-            P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
+            out_field<OOP>(P, 3)[i] = sxx; out_field<OOP>(P, 4)[i] = syy; out_field<OOP>(P, 5)[i] = szz;
+            out_field<OOP>(P, 6)[i] = sxy; out_field<OOP>(P, 7)[i] = syz; out_field<OOP>(P, 8)[i] = sxz;
             bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) ||
                    !isfinite(sxz);
             // Free-surface ghost planes (stressGhostKernel3D, cuda/Cuda_Seis3D.cu:203-219) from
@@ -565,11 +577,11 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             if (g.free_surface && z <= 8) {
                 if (z >= 5) {
                     const long long gi = dcell(g, x, y, 8 - z);
-                    P.p.f[5][gi] = -szz; P.p.f[3][gi] = sxx; P.p.f[4][gi] = syy; P.p.f[6][gi] = sxy;
+                    out_field<OOP>(P, 5)[gi] = -szz; out_field<OOP>(P, 3)[gi] = sxx; out_field<OOP>(P, 4)[gi] = syy; out_field<OOP>(P, 6)[gi] = sxy;
                 }
                 if (z <= 7) {
                     const long long gi = dcell(g, x, y, 7 - z);
-                    P.p.f[8][gi] = -sxz; P.p.f[7][gi] = -syz;
+                    out_field<OOP>(P, 8)[gi] = -sxz; out_field<OOP>(P, 7)[gi] = -syz;
                 }
             }
         }
@@ -582,7 +594,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             d00 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[0] + cx[0 * 64 + tx] * d00;
-                P.p.psi[0][psi_index_x(g, 3, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 3, qx, y, k)] = m;
                 d00 = cx[2 * 64 + tx] * d00 + m;
             }
 #pragma unroll
@@ -590,7 +602,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float d01 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[0] + cyt[3 * BY + ty] * d01;
-                P.p.psi[1][psi_index_y(g, 3, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 3, qy, x, k)] = m;
                 d01 = cyt[5 * BY + ty] * d01 + m;
             }
             // vy: D+x (slot 4, half), D-y (slot 4, integer).
@@ -599,7 +611,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float d10 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[1] + cx[3 * 64 + tx] * d10;
-                P.p.psi[0][psi_index_x(g, 4, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 4, qx, y, k)] = m;
                 d10 = cx[5 * 64 + tx] * d10 + m;
             }
 #pragma unroll
@@ -607,7 +619,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             d11 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[1] + cyt[0 * BY + ty] * d11;
-                P.p.psi[1][psi_index_y(g, 4, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 4, qy, x, k)] = m;
                 d11 = cyt[2 * BY + ty] * d11 + m;
             }
             s01 = d01 + d10;
@@ -617,7 +629,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             d20 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[2] + cx[3 * 64 + tx] * d20;
-                P.p.psi[0][psi_index_x(g, 5, qx, y, k)] = m;
+                out_psi<OOP>(P, 0)[psi_index_x(g, 5, qx, y, k)] = m;
                 d20 = cx[5 * 64 + tx] * d20 + m;
             }
 #pragma unroll
@@ -625,7 +637,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             d21 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[2] + cyt[3 * BY + ty] * d21;
-                P.p.psi[1][psi_index_y(g, 5, qy, x, k)] = m;
+                out_psi<OOP>(P, 1)[psi_index_y(g, 5, qy, x, k)] = m;
                 d21 = cyt[5 * BY + ty] * d21 + m;
             }
         }
@@ -648,11 +660,11 @@ hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
     if (tile_y == 4) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<4>, 64 * 4, 0);
-        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<4>, 64 * 4, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<4, false>, 64 * 4, 0);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<4, false>, 64 * 4, 0);
     } else if (tile_y == 8) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<8>, 64 * 8, 0);
-        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<8>, 64 * 8, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma<8, false>, 64 * 8, 0);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma<8, false>, 64 * 8, 0);
     } else {
         return hipErrorInvalidValue;
     }
@@ -671,14 +683,25 @@ hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, h
     const int launched = (m.nblocks + 7) / 8 * 8;
     m.per_xcd = launched / 8;
     const dim3 grid(launched);
+    const bool oop = out_of_place(P.p);
     switch (tile_y) {
     case 4:
-        if (velocity) hipLaunchKernelGGL(velocity_dma<4>, grid, dim3(64, 4), 0, s, P, m);
-        else hipLaunchKernelGGL(stress_dma<4>, grid, dim3(64, 4), 0, s, P, m);
+        if (oop) {
+            if (velocity) hipLaunchKernelGGL((velocity_dma<4, true>), grid, dim3(64, 4), 0, s, P, m);
+            else hipLaunchKernelGGL((stress_dma<4, true>), grid, dim3(64, 4), 0, s, P, m);
+        } else {
+            if (velocity) hipLaunchKernelGGL((velocity_dma<4, false>), grid, dim3(64, 4), 0, s, P, m);
+            else hipLaunchKernelGGL((stress_dma<4, false>), grid, dim3(64, 4), 0, s, P, m);
+        }
         break;
     case 8:
-        if (velocity) hipLaunchKernelGGL(velocity_dma<8>, grid, dim3(64, 8), 0, s, P, m);
-        else hipLaunchKernelGGL(stress_dma<8>, grid, dim3(64, 8), 0, s, P, m);
+        if (oop) {
+            if (velocity) hipLaunchKernelGGL((velocity_dma<8, true>), grid, dim3(64, 8), 0, s, P, m);
+            else hipLaunchKernelGGL((stress_dma<8, true>), grid, dim3(64, 8), 0, s, P, m);
+        } else {
+            if (velocity) hipLaunchKernelGGL((velocity_dma<8, false>), grid, dim3(64, 8), 0, s, P, m);
+            else hipLaunchKernelGGL((stress_dma<8, false>), grid, dim3(64, 8), 0, s, P, m);
+        }
         break;
     default:
         return hipErrorInvalidValue;

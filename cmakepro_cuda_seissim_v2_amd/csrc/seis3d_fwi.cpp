@@ -192,8 +192,8 @@ struct Job {
 
     seis3d::DevStep forward_operands(const ShotWork& w) const {
         seis3d::DevStep P = base;
-        for (int k = 0; k < 9; ++k) P.p.f[k] = w.live.f[k];
-        for (int a = 0; a < 3; ++a) P.p.psi[a] = w.live.psi[a];
+        for (int k = 0; k < 9; ++k) P.p.f[k] = P.p.out[k] = w.live.f[k];
+        for (int a = 0; a < 3; ++a) P.p.psi[a] = P.p.psi_out[a] = w.live.psi[a];
         P.p.flags = w.flags;
         return P;
     }
@@ -206,6 +206,32 @@ struct Job {
         if (w.src.kind == SEIS3D_FORCE) V.src = make_source(c, w.src, double(n) * c.dt);
         check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
         seis3d::DevStep S = P;
+        if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
+        check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
+    }
+
+    // Step n = s0 + m of a recomputed segment, out of place, straight into the slots:
+    // from `prev` (the segment's start state for m = 0, else slot m - 1) into slot m,
+    // which holds the velocities and CPML memories after step n and, already, the
+    // stresses before it.  The stresses after step n go to slot m + 1; after the
+    // segment's last step nobody reads them and they go to the live state.  The
+    // kernels and their arithmetic are those of forward_step, without the copies.
+    // LDS-DMA kernels (family 4) only: the other families update in place.
+    void recompute_step(ShotWork& w, int64_t n, int64_t s0, int64_t s1, const StateView& start) const {
+        const std::size_t m = std::size_t(n - s0);
+        const StateView& prev = m == 0 ? start : w.slots[m - 1];
+        const StateView& cur = w.slots[m];
+        const StateView& next = n + 1 < s1 ? w.slots[m + 1] : w.live;
+        seis3d::DevStep V = base;
+        V.p.flags = w.flags;
+        for (int k = 0; k < 3; ++k) { V.p.f[k] = prev.f[k]; V.p.out[k] = cur.f[k]; }
+        for (int k = 3; k < 9; ++k) V.p.f[k] = V.p.out[k] = cur.f[k];
+        for (int a = 0; a < 3; ++a) { V.p.psi[a] = prev.psi[a]; V.p.psi_out[a] = cur.psi[a]; }
+        seis3d::DevStep S = V;
+        if (w.src.kind == SEIS3D_FORCE) V.src = make_source(c, w.src, double(n) * c.dt);
+        check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
+        for (int k = 0; k < 3; ++k) S.p.f[k] = S.p.out[k] = cur.f[k];
+        for (int k = 3; k < 9; ++k) S.p.out[k] = next.f[k];
         if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
         check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
     }
@@ -272,6 +298,11 @@ struct Job {
         for (auto* w : shots) {
             check(hipMemsetAsync(w->flags, 0, sizeof(unsigned int), w->stream), "clear FWI flags");
             zero_state(w->stream, w->live, L);
+            // Recomputed steps never write ghost cells, so every slot must hold zero ghosts.
+            // The forward pass copies whole states into the slots of the last segment; only
+            // the slots past a short last segment start unwritten.
+            for (std::size_t m = std::size_t(N - last0); m < w->slots.size(); ++m)
+                zero_state(w->stream, w->slots[m], L);
             Batcher z{w->stream, 1};
             for (auto* p : w->a) z.add(nullptr, p, L.cells);
             for (int a = 0; a < 3; ++a) z.add(nullptr, w->apsi[a], L.psi[a]);
@@ -300,7 +331,16 @@ struct Job {
         // Backward: segments in reverse, recomputing every one but the last.
         for (int j = J - 1; j >= 0; --j) {
             const int64_t s0 = int64_t(j) * K, s1 = std::min(N, s0 + K);
-            if (j != J - 1) {
+            if (j != J - 1 && t->kernels.family == 4) {
+                for (auto* w : shots) {
+                    if (j == 0) zero_state(w->stream, w->live, L);
+                    store_first_stresses(*w, j == 0 ? w->live : w->checkpoints[std::size_t(j - 1)]);
+                }
+                for (int64_t n = s0; n < s1; ++n)
+                    for (auto* w : shots)
+                        recompute_step(*w, n, s0, s1, j == 0 ? w->live : w->checkpoints[std::size_t(j - 1)]);
+            } else if (j != J - 1) {
+                // Kernel families without an out-of-place form: step the live state and copy.
                 for (auto* w : shots) {
                     if (j == 0) zero_state(w->stream, w->live, L);
                     else copy_state(w->stream, w->checkpoints[std::size_t(j - 1)], w->live);

@@ -131,9 +131,9 @@ struct seis3d_session {
         P.g.free_surface = c.free_surface;
         P.g.tz = (c.free_surface ? 1 : 2) * P.g.pml;
         for (int a = 0; a < 3; ++a) P.g.slab[a] = static_cast<long long>(plan.slab[a]);
-        for (int f = 0; f < 9; ++f) P.p.f[f] = fields[f];
+        for (int f = 0; f < 9; ++f) P.p.f[f] = P.p.out[f] = fields[f];
         P.p.inv_rho = inv_rho; P.p.lambda = lambda; P.p.mu = mu; P.p.inv_mu = inv_mu;
-        for (int a = 0; a < 3; ++a) { P.p.psi[a] = psi[a]; P.p.coef[a] = coef[a]; }
+        for (int a = 0; a < 3; ++a) { P.p.psi[a] = P.p.psi_out[a] = psi[a]; P.p.coef[a] = coef[a]; }
         P.p.flags = flags;
         P.dt = c.dt;
         P.invh = 1.0f / c.spacing;

@@ -775,6 +775,7 @@ static hipError_t stream_launch(bool velocity, const DevStep& P, int tile_y, int
 
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s) {
     if (k.family == 4) return launch_dma(true, P, k.tile_y, k.z_chunk, s);
+    if (out_of_place(P.p)) return hipErrorInvalidValue;   // in-place families only
     if (k.family == 3) return launch_column(true, P, k.tile_y, k.z_chunk, s);
     if (k.family == 2) return launch_pipe(true, P, k.tile_y, k.z_chunk, s);
     if (k.family == 1) return stream_launch(true, P, k.tile_y, k.z_chunk, s);
@@ -785,6 +786,7 @@ hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t 
 
 hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s) {
     if (k.family == 4) return launch_dma(false, P, k.tile_y, k.z_chunk, s);
+    if (out_of_place(P.p)) return hipErrorInvalidValue;
     if (k.family == 3) return launch_column(false, P, k.tile_y, k.z_chunk, s);
     if (k.family == 2) return launch_pipe(false, P, k.tile_y, k.z_chunk, s);
     if (k.family == 1) return stream_launch(false, P, k.tile_y, k.z_chunk, s);

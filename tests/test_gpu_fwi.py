@@ -4,8 +4,10 @@ No reference counterpart. Pinned by
 * the FP64 and FP32 oracle adjoints on the same seeded cases (the FP64 one is itself pinned by
   central differences in test_oracle_adjoint.py);
 * central differences of the GPU's own forward misfit along seeded directions;
-* invariances: the checkpoint interval (1, 7, all steps) changes nothing, bit for bit; a batched
-  request equals the sum of single-shot requests; the session's own wavefield is untouched.
+* invariances: the checkpoint interval (1, 7, all steps) changes nothing, bit for bit; the
+  out-of-place recompute of the LDS-DMA kernels equals the copy route of the in-place kernel
+  families bit for bit; a batched request equals the sum of single-shot requests; the
+  session's own wavefield is untouched.
 """
 import numpy as np
 import pytest
@@ -95,6 +97,24 @@ def test_checkpoint_interval_is_bitwise_neutral(gpu, small):
         assert got[0] == ref[0]
         for a, b in zip(got[1:], ref[1:]):
             assert np.array_equal(a, b), interval
+
+
+@pytest.mark.parametrize("family,tile_y,z_chunk,interval", [(0, 4, 16, 7), (3, 4, 16, 7), (0, 4, 16, 9)])
+def test_out_of_place_recompute_equals_copy_path(gpu, small, family, tile_y, z_chunk, interval):
+    """The LDS-DMA kernels step recomputed segments out of place, straight into the stored
+    states; the in-place kernel families step a live state and copy it after every step.
+    The families are bitwise-identical, so both routes give the same gradient bit for bit.
+    Interval 9 leaves a short last segment (70 = 7 * 9 + 7), whose unused slots the
+    out-of-place route must clear itself."""
+    results = []
+    for fam, ty, zc in ((4, 4, 16), (family, tile_y, z_chunk)):
+        with S.GPUSimData3D(small["ctx"]) as g:
+            g.upload_model(*small["start"])
+            g.set_tiles(fam, ty, zc)
+            results.append(g.fwi_gradient(small["shots"], STEPS, small["stride"], interval))
+    assert results[0][0] == results[1][0]
+    for a, b in zip(results[0][1:], results[1][1:]):
+        assert np.array_equal(a, b)
 
 
 def test_batched_request_equals_single_shots(gpu):
