@@ -89,6 +89,131 @@ __device__ __forceinline__ const float* coef_row(const AdjStep& P, int axis, int
 }
 
 // ---------------------------------------------------------------------------
+// 2.5D plane march of the gather passes
+// ---------------------------------------------------------------------------
+// A 64 x 4 workgroup owns an x-y tile of the updated cells and marches a z-chunk.
+// A gather differentiates, for each component v, one field along x, one along y and
+// one along z (its "triples").  The z-fields live in 9-deep register queues along the
+// thread's own column, so each plane is read once instead of eight times; each plane's
+// x- and y-fields are staged in LDS with their 4-cell halos; the next plane's loads
+// are in flight while the current plane computes.  The local passes stay one thread
+// per cell: their many per-cell loads (CPML memories, gradient sums, materials) hide
+// behind 8 waves per SIMD, and marched they ran at half that speed.
+constexpr int kMarchChunk = 16;
+
+struct Triples {
+    const float* x[3];
+    const float* y[3];
+    const float* z[3];
+};
+
+struct MarchTile {
+    float sx[3][4][72];    // x-fields on rows y0..y0+3, columns x0-4..x0+67
+    float sy[3][12][64];   // y-fields on rows y0-4..y0+7, columns x0..x0+63
+};
+
+// One thread's share of a plane's staged fields, held in registers between the load
+// and the LDS store.
+struct MarchStage {
+    float x[3][2];   // columns x0-4+tx and x0+60+tx (the latter for tx < 8)
+    float y[3][3];   // rows y0-4+ty, y0+ty, y0+4+ty
+};
+
+__device__ __forceinline__ void load_plane(const DevGrid& g, const Triples& F, MarchStage& S, int x0, int y0,
+                                           long long plane) {
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+    const int yr = y0 + ty;
+    const long long row = plane + static_cast<long long>(yr) * g.px;
+    const int xa = x0 - 4 + tx, xb = x0 + 60 + tx, xc = x0 + tx;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        S.x[v][0] = yr < g.gy && xa < g.gx ? F.x[v][row + xa] : 0.0f;
+        S.x[v][1] = tx < 8 && yr < g.gy && xb < g.gx ? F.x[v][row + xb] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int yc = y0 - 4 + ty + 4 * k;
+            S.y[v][k] = yc < g.gy && xc < g.gx ? F.y[v][plane + static_cast<long long>(yc) * g.px + xc] : 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void store_plane(MarchTile& T, const MarchStage& S) {
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        T.sx[v][ty][tx] = S.x[v][0];
+        if (tx < 8) T.sx[v][ty][64 + tx] = S.x[v][1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) T.sy[v][ty + 4 * k][tx] = S.y[v][k];
+    }
+}
+
+// Derivatives of the staged plane at the thread's cell.
+struct PlaneStencils {
+    const MarchTile& T;
+    const float (&q)[3][9];   // z-fields at z-4 .. z+4
+    float invh;
+    // Triple field (v, axis a): D+ (taps -3..4) when plus, else D- (taps -4..3).
+    __device__ __forceinline__ float operator()(int v, int a, bool plus) const {
+        const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+        const int off = plus ? 3 : 4;
+        float s8[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (a == 0) s8[j] = T.sx[v][ty][tx + 4 + j - off];
+            else if (a == 1) s8[j] = T.sy[v][ty + 4 + j - off][tx];
+            else s8[j] = q[v][j + 4 - off];
+        }
+        return stencil8(s8, invh);
+    }
+};
+
+// Runs body(cell, stencils) at every updated cell of the workgroup's tile and z-chunk.
+template <class Body>
+__device__ __forceinline__ void march(const AdjStep& P, const TileMap& M, const Triples& F, MarchTile& T,
+                                      Body&& body) {
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    const DevGrid& g = P.g;
+    const int x0 = 4 + bx * 64, y0 = 4 + by * 4;
+    Cell c;
+    c.x = x0 + int(threadIdx.x);
+    c.y = y0 + int(threadIdx.y);
+    const bool own = c.x < g.gx - 4 && c.y < g.gy - 4;
+    c.q[0] = slab_q(c.x, g.gx, g.pml, false);
+    c.q[1] = slab_q(c.y, g.gy, g.pml, false);
+    int z0, z1;
+    chunk_of(M, g.gz, bz, z0, z1);
+    const long long col = static_cast<long long>(c.y) * g.px + c.x;
+    float q[3][9], lead[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[v][j + 1] = own ? F.z[v][col + static_cast<long long>(z0 - 4 + j) * g.plane] : 0.0f;
+        lead[v] = own ? F.z[v][col + static_cast<long long>(z0 + 4) * g.plane] : 0.0f;
+    }
+    MarchStage S;
+    load_plane(g, F, S, x0, y0, static_cast<long long>(z0) * g.plane);
+    for (int z = z0; z < z1; ++z) {
+        const long long plane = static_cast<long long>(z) * g.plane;
+        c.z = z;
+        c.i = plane + col;
+        c.q[2] = slab_q(z, g.gz, g.pml, g.free_surface != 0);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) shift(q[v], lead[v]);
+        store_plane(T, S);
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) lead[v] = own ? F.z[v][c.i + 5 * g.plane] : 0.0f;
+            load_plane(g, F, S, x0, y0, plane + g.plane);
+        }
+        __syncthreads();
+        if (own) body(c, PlaneStencils{T, q, P.invh});
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Stress half-step adjoint
 // ---------------------------------------------------------------------------
 
@@ -142,137 +267,27 @@ __global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
         }
 }
 
-// 2.5D gather passes: a 64 x 4 workgroup owns an x-y tile and marches a z-chunk.  The
-// z-stencil temporaries t[3v+2] live in 9-deep register queues along the own column
-// (each plane read once instead of eight times); each plane's x- and y-stencil
-// temporaries t[3v], t[3v+1] are staged in LDS with their 4-cell halos.
-constexpr int kGatherChunk = 16;
-
-struct GatherTile {
-    float sx[3][4][72];    // t[3v] on rows y0..y0+3, columns x0-4..x0+67
-    float sy[3][12][64];   // t[3v+1] on rows y0-4..y0+7, columns x0..x0+63
-};
-
-// One thread's share of a plane's staged temporaries, held in registers so that the
-// next plane's loads are in flight while the current plane computes.
-struct GatherStage {
-    float x[3][2];   // t[3v] at columns x0-4+tx and x0+60+tx (the latter for tx < 8)
-    float y[3][3];   // t[3v+1] on rows y0-4+ty, y0+ty, y0+4+ty
-};
-
-__device__ __forceinline__ void load_gather_plane(const AdjStep& P, GatherStage& S, int x0, int y0, long long plane) {
-    const DevGrid& g = P.g;
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    const int yr = y0 + ty;
-    const long long row = plane + static_cast<long long>(yr) * g.px;
+// Gather pass: L_v -= D^T-stencils of L_d (D- for a == v was used forward, so D+ here).
+__global__ __launch_bounds__(256) void adjoint_stress_gather(AdjStep P, TileMap M) {
+    __shared__ MarchTile T;
+    const Triples F{{P.t[0], P.t[3], P.t[6]}, {P.t[1], P.t[4], P.t[7]}, {P.t[2], P.t[5], P.t[8]}};
+    march(P, M, F, T, [&](const Cell& c, const PlaneStencils& D) {
 #pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        const float* fx = P.t[3 * v];
-        const float* fy = P.t[3 * v + 1];
-        const int xa = x0 - 4 + tx, xb = x0 + 60 + tx;
-        S.x[v][0] = yr < g.gy && xa < g.gx ? fx[row + xa] : 0.0f;
-        S.x[v][1] = tx < 8 && yr < g.gy && xb < g.gx ? fx[row + xb] : 0.0f;
+        for (int v = 0; v < 3; ++v) {
+            float acc = P.a[v][c.i];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int yc = y0 - 4 + ty + 4 * k, xc = x0 + tx;
-            S.y[v][k] = yc < g.gy && xc < g.gx ? fy[plane + static_cast<long long>(yc) * g.px + xc] : 0.0f;
+            for (int a = 0; a < 3; ++a) acc -= D(v, a, a == v);
+            P.a[v][c.i] = acc;
         }
-    }
-}
-
-__device__ __forceinline__ void store_gather_plane(GatherTile& T, const GatherStage& S) {
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        T.sx[v][ty][tx] = S.x[v][0];
-        if (tx < 8) T.sx[v][ty][64 + tx] = S.x[v][1];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) T.sy[v][ty + 4 * k][tx] = S.y[v][k];
-    }
-}
-
-// Transposed stencil of t[3v+a] at the thread's cell: D+ (taps -3..4) when plus, else D-.
-__device__ __forceinline__ float gather_stencil(const AdjStep& P, const GatherTile& T, const float (&q)[9], int v,
-                                                int a, bool plus) {
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    const int off = plus ? 3 : 4;
-    float s8[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        if (a == 0) s8[j] = T.sx[v][ty][tx + 4 + j - off];
-        else if (a == 1) s8[j] = T.sy[v][ty + 4 + j - off][tx];
-        else s8[j] = q[j + 4 - off];
-    }
-    return stencil8(s8, P.invh);
-}
-
-template <bool VELOCITY>
-__global__ __launch_bounds__(256) void adjoint_gather_25d(AdjStep P, TileMap M) {
-    __shared__ GatherTile T;
-    int bx, by, bz;
-    if (!tile_of(M, bx, by, bz)) return;
-    const DevGrid& g = P.g;
-    const int x0 = 4 + bx * 64, y0 = 4 + by * 4;
-    const int x = x0 + int(threadIdx.x), y = y0 + int(threadIdx.y);
-    const bool own = x < g.gx - 4 && y < g.gy - 4;
-    int z0, z1;
-    chunk_of(M, g.gz, bz, z0, z1);
-    const long long col = static_cast<long long>(y) * g.px + x;
-    float q[3][9], lead[3];
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-        const float* tz = P.t[3 * v + 2];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) q[v][j + 1] = own ? tz[col + static_cast<long long>(z0 - 4 + j) * g.plane] : 0.0f;
-        lead[v] = own ? tz[col + static_cast<long long>(z0 + 4) * g.plane] : 0.0f;
-    }
-    GatherStage S;
-    load_gather_plane(P, S, x0, y0, static_cast<long long>(z0) * g.plane);
-    for (int z = z0; z < z1; ++z) {
-        const long long plane = static_cast<long long>(z) * g.plane;
-        const long long i = plane + col;
-#pragma unroll
-        for (int v = 0; v < 3; ++v) shift(q[v], lead[v]);
-        store_gather_plane(T, S);
-        if (z + 1 < z1) {
-            // Next plane's loads, in flight while this plane computes.
-#pragma unroll
-            for (int v = 0; v < 3; ++v) lead[v] = own ? P.t[3 * v + 2][i + 5 * g.plane] : 0.0f;
-            load_gather_plane(P, S, x0, y0, plane + g.plane);
-        }
-        __syncthreads();
-        if (own) {
-            if (!VELOCITY) {
-                // L_v -= D^T-stencils of L_d (D- for a == v was used forward, so D+ here).
-#pragma unroll
-                for (int v = 0; v < 3; ++v) {
-                    float acc = P.a[v][i];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) acc -= gather_stencil(P, T, q[v], v, a, a == v);
-                    P.a[v][i] = acc;
-                }
-            } else {
-                // L_sigma -= transposed stencils of L_d, in the (v, a) order of the forward sum.
-                const int sf[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
-                float s[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) s[k] = P.a[3 + k][i];
-#pragma unroll
-                for (int v = 0; v < 3; ++v)
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) s[sf[v][a]] -= gather_stencil(P, T, q[v], v, a, a != v);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) P.a[3 + k][i] = s[k];
-            }
-        }
-        __syncthreads();
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
 // Velocity half-step adjoint
 // ---------------------------------------------------------------------------
 
+// Local pass: stress divergence of the stored stresses (and the force source) for the
+// buoyancy gradients, and the filtered adjoint derivatives L_d[v][a] of dt h L_v.
 __global__ __launch_bounds__(256) void adjoint_velocity_local(AdjStep P) {
     Cell c;
     if (!updated_cell(P.g, c)) return;
@@ -305,6 +320,24 @@ __global__ __launch_bounds__(256) void adjoint_velocity_local(AdjStep P) {
             P.t[3 * v + a][c.i] = ld;
         }
     }
+}
+
+// Gather pass: L_sigma -= transposed stencils of L_d, in the (v, a) order of the forward sum.
+__global__ __launch_bounds__(256) void adjoint_velocity_gather(AdjStep P, TileMap M) {
+    __shared__ MarchTile T;
+    const Triples F{{P.t[0], P.t[3], P.t[6]}, {P.t[1], P.t[4], P.t[7]}, {P.t[2], P.t[5], P.t[8]}};
+    march(P, M, F, T, [&](const Cell& c, const PlaneStencils& D) {
+        const int sf[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+        float s[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] = P.a[3 + k][c.i];
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s[sf[v][a]] -= D(v, a, a != v);   // D- for a == v, D+ otherwise
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.a[3 + k][c.i] = s[k];
+    });
 }
 
 // R^T r: the receiver collocation 0.5 (v[i] + v[i - e_v]) transposed.
@@ -465,19 +498,20 @@ hipError_t launch_arrays(const ArrayBatch& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-// 64 x 4 tiles of the updated cells and z-chunks of at most kGatherChunk planes, dealt to
+// 64 x 4 tiles of the updated cells and z-chunks of at most kMarchChunk planes, dealt to
 // the XCDs by tile_of.
-TileMap gather_tiles(const DevGrid& g) {
+TileMap march_tiles(const DevGrid& g) {
     TileMap m;
     m.nbx = (g.gx - 8 + 63) / 64;
     m.nby = (g.gy - 8 + 3) / 4;
-    m.zchunk = kGatherChunk;
-    m.nbz = (g.gz - 8 + kGatherChunk - 1) / kGatherChunk;
+    m.zchunk = kMarchChunk;
+    m.nbz = (g.gz - 8 + kMarchChunk - 1) / kMarchChunk;
     m.nblocks = m.nbx * m.nby * m.nbz;
     m.per_xcd = (m.nblocks + 7) / 8;
     return m;
 }
 
+// One thread per updated cell: a 64 x 4 tile of one plane per workgroup.
 dim3 updated_grid(const DevGrid& g) {
     return dim3(unsigned((g.gx - 8 + 63) / 64), unsigned((g.gy - 8 + 3) / 4), unsigned(g.gz - 8));
 }
@@ -485,16 +519,18 @@ dim3 updated_grid(const DevGrid& g) {
 }  // namespace
 
 hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s) {
-    const TileMap m = gather_tiles(P.g);
-    hipLaunchKernelGGL(adjoint_stress_local, updated_grid(P.g), dim3(64, 4), 0, s, P);
-    hipLaunchKernelGGL(adjoint_gather_25d<false>, dim3(unsigned(m.per_xcd * 8)), dim3(64, 4), 0, s, P, m);
+    const TileMap m = march_tiles(P.g);
+    const dim3 grid(unsigned(m.per_xcd * 8)), block(64, 4);
+    hipLaunchKernelGGL(adjoint_stress_local, updated_grid(P.g), block, 0, s, P);
+    hipLaunchKernelGGL(adjoint_stress_gather, grid, block, 0, s, P, m);
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s) {
-    const TileMap m = gather_tiles(P.g);
-    hipLaunchKernelGGL(adjoint_velocity_local, updated_grid(P.g), dim3(64, 4), 0, s, P);
-    hipLaunchKernelGGL(adjoint_gather_25d<true>, dim3(unsigned(m.per_xcd * 8)), dim3(64, 4), 0, s, P, m);
+    const TileMap m = march_tiles(P.g);
+    const dim3 grid(unsigned(m.per_xcd * 8)), block(64, 4);
+    hipLaunchKernelGGL(adjoint_velocity_local, updated_grid(P.g), block, 0, s, P);
+    hipLaunchKernelGGL(adjoint_velocity_gather, grid, block, 0, s, P, m);
     return hipGetLastError();
 }
 
