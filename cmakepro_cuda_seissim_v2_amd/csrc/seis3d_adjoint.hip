@@ -99,7 +99,7 @@ __device__ __forceinline__ const float* coef_row(const AdjStep& P, int axis, int
 // are in flight while the current plane computes.  The local passes stay one thread
 // per cell: their many per-cell loads (CPML memories, gradient sums, materials) hide
 // behind 8 waves per SIMD, and marched they ran at half that speed.
-constexpr int kMarchChunk = 16;
+constexpr int kMarchChunk = 12;
 
 struct Triples {
     const float* x[3];
