@@ -51,7 +51,7 @@ __device__ __forceinline__ void count_step(const Dev2D& g) {
 
 // updateType1StressKernel (:225-266) and, with `fsz`, the interior rows of
 // updateType3StressKernel (:465-485), whose z damping is off near the surface.
-__device__ __forceinline__ void split_stress(const Dev2D& g, int i, int j, int k, float dzv, float dzhv) {
+__device__ __forceinline__ void split_stress(const Dev2D& g, int j, int k, float dzv, float dzhv) {
     float* const* f = g.f;
     const float dvx_dx = fwd(g, f[0], f[1], k, 1), dvz_dz = bwd(g, f[2], f[3], k, g.nx);
     const float l2 = 0.5f * (g.l2m[k] + g.l2m[k + 1]), la = 0.5f * (g.lambda[k] + g.lambda[k + 1]);
@@ -66,7 +66,7 @@ __device__ __forceinline__ void split_stress(const Dev2D& g, int i, int j, int k
     f[9][k] = pml(f[9][k], g.inv_dt, dzhv, m * dvx_dz);
 }
 
-__device__ __forceinline__ void split_velocity(const Dev2D& g, int i, int j, int k, float dzv, float dzhv) {
+__device__ __forceinline__ void split_velocity(const Dev2D& g, int j, int k, float dzv, float dzhv) {
     float* const* f = g.f;
     const float r = g.rho[k];
     const float sxx_x = bwd(g, f[4], f[5], k, 1), sxz_z = bwd(g, f[8], f[9], k, g.nx);
@@ -82,13 +82,13 @@ __global__ __launch_bounds__(256) void stress_type1(Dev2D g) {
     count_step(g);
     int i, j;
     if (!cell(g, i, j, kR)) return;
-    split_stress(g, i, j, i * g.nx + j, g.dz[i], g.dzh[i]);
+    split_stress(g, j, i * g.nx + j, g.dz[i], g.dzh[i]);
 }
 
 __global__ __launch_bounds__(256) void velocity_type1(Dev2D g) {
     int i, j;
     if (!cell(g, i, j, kR)) return;
-    split_velocity(g, i, j, i * g.nx + j, g.dz[i], g.dzh[i]);
+    split_velocity(g, j, i * g.nx + j, g.dz[i], g.dzh[i]);
 }
 
 // updateType2StressKernel (:315-360): plain update inside, split PML in the frame.
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void stress_type3(Dev2D g) {
     const bool near = i <= g.fs + 5 && g.up;
     const float dzv = near ? 0.0f : g.dz[i], dzhv = near ? 0.0f : g.dzh[i];
     if (i != g.fs) {
-        split_stress(g, i, j, k, dzv, dzhv);
+        split_stress(g, j, k, dzv, dzhv);
         return;
     }
     float* const* f = g.f;
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void velocity_type3(Dev2D g) {
         return;
     }
     const bool near = i <= g.fs + 5 && g.up;
-    split_velocity(g, i, j, k, near ? 0.0f : g.dz[i], near ? 0.0f : g.dzh[i]);
+    split_velocity(g, j, k, near ? 0.0f : g.dz[i], near ? 0.0f : g.dzh[i]);
 }
 
 // applyPointSource (:567-636); `atomic` for overlapping multi-source injection.
