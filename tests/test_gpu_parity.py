@@ -44,6 +44,17 @@ CASES = {
         lambda: W.imported_context(30, 26, 28, max_vp=4400.0, pml=10, frequency=20.0),
         dict(x=11, y=14, z=6, kind=S.FORCE, direction=(0.48, -0.6, 0.64)),
         lambda c: W.gradient_random_model(c.nx, c.ny, c.nz, seed=4, corr=4.0), 140, 3),
+    # The reference's limits at once: the smallest grid (16 per axis), a source at the smallest
+    # valid position next to the free surface (x = y = 1, z = 2; validateContext3D / :604-607).
+    "min-grid-edge-explosion": (
+        lambda: W.imported_context(16, 16, 16, max_vp=4500.0, pml=8, frequency=30.0),
+        dict(x=1, y=1, z=2, kind=S.EXPLOSION),
+        lambda c: W.layered_model(c.nx, c.ny, c.nz, seed=6), 100, 1),
+    # The thickest CPML (40 cells), the largest CFL (0.9) and a force at the far corner.
+    "max-pml-cfl-corner-force": (
+        lambda: S.validateContext3D(S.default_context(nx=20, ny=18, nz=24, pml=40, free_surface=0, cfl=0.9,
+                                                      model_kind=S.DIPPING_LAYER, frequency=20.0)),
+        dict(x=18, y=16, z=22, kind=S.FORCE, direction=(-0.6, 0.48, 0.64)), None, 100, 2),
     "reflecting-uniform-explosion": (
         lambda: S.validateContext3D(S.default_context(nx=24, ny=20, nz=22, pml=12, reflecting=1,
                                                       model_kind=S.UNIFORM)),
@@ -127,6 +138,50 @@ def test_gather_and_snapshot_parity(gpu, oracle_runs, name, tiles):
         s64 = rel_l2(snap[core], f64[f][core])
         assert s32 <= TOL32, f"field {f} rel-L2 vs FP32 oracle {s32:.3e}"
         assert s64 <= TOL64, f"field {f} rel-L2 vs FP64 oracle {s64:.3e}"
+    g.release()
+
+
+GENERATED = [S.UNIFORM, S.TWO_LAYER, S.SPHERE, S.THREE_LAYER, S.DIPPING_LAYER, S.LOW_VELOCITY_LAYER, S.FAULT,
+             S.GRADIENT, S.SALT_DOME]
+
+
+def generated_context(kind, free_surface=1, **kw):
+    """A generated model (generateMaterialsKernel3D, cuda/Cuda_Seis3D.cu:114-147) with non-default
+    interface, dip, thickness, throw and radius so every branch of its kind is exercised."""
+    return S.validateContext3D(S.default_context(
+        nx=40, ny=30, nz=36, pml=8, free_surface=free_surface, model_kind=kind, vp=2800.0, vs=1600.0,
+        density=2100.0, layer_vp=3900.0, layer_vs=2200.0, layer_density=2450.0, interface_depth=0.45,
+        layer_dip=0.3, layer_thickness=0.2, fault_throw=0.15, anomaly_radius=0.27, **kw))
+
+
+@pytest.mark.parametrize("kind", GENERATED)
+def test_generated_materials_bit_exact(gpu, kind):
+    """1/rho, lambda and mu of every generated model kind, padding (clamp extension) included,
+    equal the oracle's bit for bit: both sides use the same elementary operations without
+    contraction."""
+    c = generated_context(kind)
+    g = S.GPUSimData3D(c)
+    with O.Oracle3D(c, 32) as o:
+        for field in (S.INV_RHO, S.LAMBDA, S.MU):
+            got = g.download_field(field)
+            want = o.field(field).astype(np.float32)
+            diff = int(np.sum(got != want))
+            assert diff == 0, f"model kind {kind} field {field}: {diff} cells differ"
+    g.release()
+
+
+@pytest.mark.parametrize("kind", [S.FAULT, S.SALT_DOME])
+def test_generated_model_gather_parity(gpu, kind):
+    """A force shot through the fault and salt-dome models: gather within the FP32 tolerance."""
+    c = generated_context(kind, frequency=25.0)
+    src = S.default_source(x=20, y=15, z=4, kind=S.FORCE, direction=(0.3, 0.0, 0.95))
+    rec = receivers_for(c)
+    steps, stride = 120, 2
+    g, gather = run_gpu(c, src, None, steps, stride, rec, (4, 4, 16))
+    o, g32 = run_oracle(c, src, None, steps, stride, 32, rec)
+    e32 = rel_l2(gather, g32)
+    log(f"model kind {kind}: gather rel-L2 vs fp32 oracle {e32:.3e}")
+    assert gather.shape == g32.shape and e32 <= TOL32, f"gather rel-L2 vs FP32 oracle {e32:.3e}"
     g.release()
 
 
