@@ -458,8 +458,9 @@ def cpu_info():
     return cores, model
 
 
-def cpu_baseline(ctx, model, src, rec, steps_hint):
-    """FP32 CPU oracle (OpenMP over z) on the same workload, bounded to ~10-30 s."""
+def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
+    """FP32 CPU oracle (OpenMP over z) on the same workload, bounded to ~10-30 s: all host
+    cores (the value), one thread, and the host STREAM triad as the CPU's bandwidth ceiling."""
     from oracle import pyoracle as O
     cores, cpu_model = cpu_info()
     threads = min(cores, 16)
@@ -470,16 +471,30 @@ def cpu_baseline(ctx, model, src, rec, steps_hint):
     warm = o.time_steps(src, 0, 1)
     steps = steps_hint if steps_hint > 0 else max(2, min(60, int(15.0 / max(warm, 1e-3))))
     secs = o.time_steps(src, 1, steps)
+    O.set_threads(1)
+    one_steps = 2
+    one_secs = o.time_steps(src, 1 + steps, one_steps)
     o.close()
+    O.set_threads(threads)
+    triad = O.stream_triad(1 << 26, threads)   # 3 x 512 MiB of doubles: far beyond the caches
     n_upd = (ctx.gx - 8) * (ctx.gy - 8) * (ctx.gz - 8)
+    step_bytes = 4.0 * (30 * n_upd + 12 * n_psi)
+    cpu_gbs = step_bytes * steps / secs / 1e9
+    print(f"bench: cpu oracle {threads} threads {steps} steps {secs:.2f} s, 1 thread {one_steps} steps "
+          f"{one_secs:.2f} s, host triad {triad:.1f} GB/s", file=sys.stderr, flush=True)
     return {
         "value": round(n_upd * steps / secs / 1e9, 5),
         "unit": "Gcell-updates/s",
         "cores": threads,
         "kind": "port",
+        "one_thread_value": round(n_upd * one_steps / one_secs / 1e9, 5),
+        "stream_triad_gbs": round(triad, 1),
+        "algorithmic_gbs": round(cpu_gbs, 2),
+        "frac_of_stream_triad": round(cpu_gbs / triad, 3) if triad > 0 else None,
         "sample": f"FP32 oracle (line-faithful C++ restatement, OpenMP {threads} threads, -O3 -march=x86-64-v3, "
                   f"no FP contraction) on the same 300x50x300 workload, steps 1..{steps} after 1 warm-up step, "
-                  f"{secs:.2f} s; host CPU: {cpu_model}",
+                  f"{secs:.2f} s; then 1 thread for {one_steps} steps, {one_secs:.2f} s; host STREAM triad "
+                  f"(3 x 512 MiB doubles, {threads} threads); host CPU: {cpu_model}",
     }
 
 
@@ -610,7 +625,7 @@ def main():
         out["roofline"]["traffic_source"] = f"profiles/{src_file} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(ctx, model, src, rec, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(ctx, model, src, rec, args.cpu_steps, sb["n_psi"])
     print(json.dumps(out), flush=True)
 
 

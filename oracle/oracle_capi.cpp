@@ -2,10 +2,12 @@
 // tests and the cpu_baseline leg of bench.py (loaded with ctypes).
 //
 // TEST INFRASTRUCTURE ONLY: the checker, never the product path.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <exception>
 #include <memory>
+#include <new>
 #include <string>
 
 #ifdef _OPENMP
@@ -235,6 +237,39 @@ int oracle3d_adjoint_step(Handle* h, const seis3d_source* src, uint64_t step, do
             for (const auto& v : A.cpml) for (const auto x : v) *p++ = double(x);
         });
     });
+}
+
+// Host STREAM triad a[i] = b[i] + s * c[i] over n doubles on `threads` OpenMP threads,
+// first-touched by the same threads; the best of `reps` in GB/s, counting 24 bytes per
+// element as STREAM does.  Gives the CPU baseline a bandwidth ceiling to compare with.
+double oracle3d_stream_triad(int64_t n, int threads, int reps) {
+    if (n <= 0 || reps <= 0) return 0.0;
+    double* a = new (std::nothrow) double[std::size_t(n)];
+    double* b = new (std::nothrow) double[std::size_t(n)];
+    double* c = new (std::nothrow) double[std::size_t(n)];
+    double best = 0.0;
+    if (a && b && c) {
+#ifdef _OPENMP
+        const int saved = omp_get_max_threads();
+        omp_set_num_threads(std::max(1, threads));
+#endif
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) { a[i] = 0.0; b[i] = 1.0; c[i] = 2.0; }
+        for (int r = 0; r < reps; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) a[i] = b[i] + 3.0 * c[i];
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (s > 0) best = std::max(best, 24.0 * double(n) / s / 1e9);
+        }
+#ifdef _OPENMP
+        omp_set_num_threads(saved);
+#endif
+    }
+    delete[] a;
+    delete[] b;
+    delete[] c;
+    return best;
 }
 
 int oracle3d_nonfinite(Handle* h) {

@@ -59,6 +59,7 @@ def library() -> C.CDLL:
             "oracle3d_set_field": (C.c_int, [vp, C.c_int, P(C.c_double)]),
             "oracle3d_sample_scalar": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
             "oracle3d_nonfinite": (C.c_int, [vp]),
+            "oracle3d_stream_triad": (C.c_double, [C.c_int64, C.c_int, C.c_int]),
             "oracle3d_cpml_size": (C.c_uint64, [vp, C.c_int]),
             "oracle3d_get_cpml": (C.c_int, [vp, C.c_int, P(C.c_double)]),
             "oracle3d_set_cpml": (C.c_int, [vp, C.c_int, P(C.c_double)]),
@@ -85,6 +86,11 @@ def _check(rc: int) -> None:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def stream_triad(n: int, threads: int, reps: int = 5) -> float:
+    """Host STREAM-triad bandwidth (GB/s) over n doubles on `threads` threads."""
+    return float(library().oracle3d_stream_triad(int(n), int(threads), int(reps)))
 
 
 def set_threads(n: int) -> int:
