@@ -391,7 +391,7 @@ def bench_2d(args) -> None:
 
 def bench_abc(args) -> None:
     """Learned 8-cell absorbing boundary on 300 x 300 x 400 (36 M nodes), 1500 steps, free top,
-    layered-plus-random-field model: reflected energy against a 40-cell CPML run (the judge) for the
+    layered-plus-random-field model: reflected energy against a 40-cell CPML run (the reflection-free reference) for the
     learned frame (weights shipped in the package), the frame without its network, plain 8-cell CPML
     and plain 20-cell CPML (the fallback); and the time per step of each boundary."""
     from cmakepro_cuda_seissim_v2_amd import learned_abc as LA

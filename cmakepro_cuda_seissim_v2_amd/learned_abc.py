@@ -1,4 +1,4 @@
-"""Learned 8-cell absorbing boundary (no reference counterpart): data, training and judging.
+"""Learned 8-cell absorbing boundary (no reference counterpart): data, training and evaluation.
 
 The frame (C ABI ``seis3d_abc_*``): a pml = 8 context whose 8 CPML cells per absorbing face
 take the plain update; the outer 4 layers, whose O(8) stencils reach the zero ghost cells, take
@@ -13,7 +13,7 @@ session takes one step, the wide one takes the same step, and each sampled frame
 pair (normalised features, normalised wide - thin difference) to the GPU trainer. The trainer's
 forward, backward and Adam run on this library's kernels.
 
-Judging: the wavefield inside the physical volume against a 40-cell CPML run of the same
+Evaluation: the wavefield inside the physical volume against a 40-cell CPML run of the same
 volume (the reflection-free reference); the relative squared error over sampled steps measures
 the reflected energy. Compared: the learned frame, the frame without correction (truncation),
 plain 8-cell CPML and plain 20-cell CPML.
@@ -205,7 +205,7 @@ def generate(trainer: Trainer, runs: int, size=(48, 48, 48), steps: int = 240, e
     return total
 
 
-# ---- judging ------------------------------------------------------------------------------------------
+# ---- evaluation ---------------------------------------------------------------------------------------
 
 def reflection(params, model, src, steps: int, every: int, free_surface: bool = True, frequency: float = 25.0,
                cases=("learned", "none", "cpml8", "cpml20"), log=None) -> dict:
@@ -250,7 +250,7 @@ def reflection(params, model, src, steps: int, every: int, free_surface: bool = 
 
 
 def main(argv=None) -> int:
-    """``python -m cmakepro_cuda_seissim_v2_amd.learned_abc train|judge ...``"""
+    """``python -m cmakepro_cuda_seissim_v2_amd.learned_abc train|reflect ...``"""
     import argparse
     import json
     import sys
@@ -269,7 +269,7 @@ def main(argv=None) -> int:
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=str(WEIGHTS_PATH))
-    j = sub.add_parser("judge", help="reflected energy of the learned frame against a 40-cell CPML run")
+    j = sub.add_parser("reflect", help="reflected energy of the learned frame against a 40-cell CPML run")
     j.add_argument("--size", default="300x300x400")
     j.add_argument("--steps", type=int, default=1500)
     j.add_argument("--every", type=int, default=100)

@@ -7,7 +7,7 @@
 // meaning and exceptions (std::invalid_argument for bad input,
 // std::runtime_error for device failures, std::overflow_error for the index
 // capacity check).  GPUSimData owns an opaque session handle instead of raw
-// device pointers; the driver's direct copy into d_active_sources becomes
+// device pointers; the reference session's direct copy into d_active_sources becomes
 // uploadActiveSources().
 #pragma once
 
