@@ -354,7 +354,10 @@ def bench_2d(args) -> None:
         print(f"bench: cpu oracle {threads} threads: {n} steps in {cs:.3f} s", file=sys.stderr, flush=True)
     t_all = min(cores, 16)
     n_all, s_all = cpu[t_all]
+    O.set_threads(t_all)
+    triad = O.stream_triad(1 << 26, t_all)
     bytes_cell = 136   # type 1: stress 19 floats (4 v + 6 sigma rmw + 3 moduli), velocity 15 (6 sigma + 4 v rmw + rho)
+    cpu_gbs = bytes_cell * n_upd * n_all / s_all / 1e9
     value = n_upd * steps / secs / 1e9
     out = {
         "metric": "Gcell-updates/s 2D P-SV O(8) split-PML step (config 1, reference CPU-path workload)",
@@ -381,9 +384,14 @@ def bench_2d(args) -> None:
         },
         "cpu_baseline": {
             "value": round(n_upd * n_all / s_all / 1e9, 5), "unit": "Gcell-updates/s", "cores": t_all, "kind": "port",
+            "one_thread_value": round(n_upd * cpu[1][0] / cpu[1][1] / 1e9, 5),
+            "stream_triad_gbs": round(triad, 1),
+            "algorithmic_gbs": round(cpu_gbs, 2),
+            "frac_of_stream_triad": round(cpu_gbs / triad, 3) if triad > 0 else None,
             "sample": f"FP32 oracle (the reference's type-1 kernels restated in C++, OpenMP over rows) on config 1, "
                       f"{n_all} steps in {s_all:.2f} s on {t_all} threads; 1 thread: "
-                      f"{n_upd * cpu[1][0] / cpu[1][1] / 1e9:.5f} Gcell/s; host CPU: {cpu_model}",
+                      f"{n_upd * cpu[1][0] / cpu[1][1] / 1e9:.5f} Gcell/s; host STREAM triad "
+                      f"(3 x 512 MiB doubles, {t_all} threads); host CPU: {cpu_model}",
         },
     }
     print(json.dumps(out), flush=True)
