@@ -479,6 +479,7 @@ def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
     warm = o.time_steps(src, 0, 1)
     steps = steps_hint if steps_hint > 0 else max(2, min(60, int(15.0 / max(warm, 1e-3))))
     secs = o.time_steps(src, 1, steps)
+    snapshot = [o.field(f) for f in range(9)]   # the state after steps 0..steps, for the parity check
     O.set_threads(1)
     one_steps = 2
     one_secs = o.time_steps(src, 1 + steps, one_steps)
@@ -490,7 +491,7 @@ def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
     cpu_gbs = step_bytes * steps / secs / 1e9
     print(f"bench: cpu oracle {threads} threads {steps} steps {secs:.2f} s, 1 thread {one_steps} steps "
           f"{one_secs:.2f} s, host triad {triad:.1f} GB/s", file=sys.stderr, flush=True)
-    return {
+    baseline = {
         "value": round(n_upd * steps / secs / 1e9, 5),
         "unit": "Gcell-updates/s",
         "cores": threads,
@@ -504,6 +505,42 @@ def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
                   f"{secs:.2f} s; then 1 thread for {one_steps} steps, {one_secs:.2f} s; host STREAM triad "
                   f"(3 x 512 MiB doubles, {threads} threads); host CPU: {cpu_model}",
     }
+    return baseline, snapshot, steps + 1
+
+
+def snapshot_parity(ctx, model, src, oracle32, steps):
+    """The bench line's own error: the 9 wavefields after `steps` steps of a fresh GPU session
+    (the bench's kernels) against the FP32 oracle state of the CPU baseline and an FP64 oracle
+    run, rel-L2 over the updated cells, worst field."""
+    import numpy as np
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    from oracle import pyoracle as O
+    g = S.GPUSimData3D(ctx)
+    g.upload_model(*model)
+    g.run(src, 0, steps)
+    gpu = [g.download_field(f) for f in range(9)]
+    g.release()
+    o = O.Oracle3D(ctx, 64)
+    o.upload_model(*model)
+    o.time_steps(src, 0, steps)
+    oracle64 = [o.field(f) for f in range(9)]
+    o.close()
+    core = (slice(4, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
+
+    def worst(ref):
+        errs = []
+        for a, b in zip(gpu, ref):
+            a = np.asarray(a[core], np.float64)
+            b = np.asarray(b[core], np.float64)
+            errs.append(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+        return max(errs)
+
+    e32, e64 = worst(oracle32), worst(oracle64)
+    print(f"bench: snapshot parity after {steps} steps: rel-L2 {e32:.2e} vs FP32 oracle, {e64:.2e} vs FP64",
+          file=sys.stderr, flush=True)
+    return {"steps": steps, "fields": "vx vy vz sxx syy szz sxy syz sxz, updated cells",
+            "snapshot_rel_l2_vs_fp32_oracle": float(f"{e32:.3e}"), "tolerance_fp32": 1e-5,
+            "snapshot_rel_l2_vs_fp64_oracle": float(f"{e64:.3e}"), "tolerance_fp64": 1e-4}
 
 
 def main():
@@ -633,7 +670,8 @@ def main():
         out["roofline"]["traffic_source"] = f"profiles/{src_file} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(ctx, model, src, rec, args.cpu_steps, sb["n_psi"])
+        out["cpu_baseline"], snapshot, snap_steps = cpu_baseline(ctx, model, src, rec, args.cpu_steps, sb["n_psi"])
+        out["parity"] = snapshot_parity(ctx, model, src, snapshot, snap_steps)
     print(json.dumps(out), flush=True)
 
 
