@@ -366,6 +366,19 @@ def bench_2d(args) -> None:
     n_all, s_all = cpu[t_all]
     O.set_threads(t_all)
     triad = O.stream_triad(1 << 26, t_all)
+    # The bench line's own error: the timed GPU run's receiver records against the oracles.
+    parity = {"steps": steps, "records": "vx, vz of every receiver at every step"}
+    for precision in (32, 64):
+        with O2.Oracle2D(ctx, precision) as o:
+            rx, rz = o.run(0, steps)
+        got = np.concatenate([vx.astype(np.float64).ravel(), vz.astype(np.float64).ravel()])
+        ref = np.concatenate([rx.ravel(), rz.ravel()])
+        err = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+        parity[f"rel_l2_vs_fp{precision}_oracle"] = float(f"{err:.3e}")
+        if precision == 32:
+            parity["bit_equal_vs_fp32_oracle"] = bool(np.array_equal(got.astype(np.float32), ref.astype(np.float32)))
+    parity["tolerance_fp32"], parity["tolerance_fp64"] = 1e-5, 1e-4
+    print(f"bench: 2d parity {parity}", file=sys.stderr, flush=True)
     bytes_cell = 136   # type 1: stress 19 floats (4 v + 6 sigma rmw + 3 moduli), velocity 15 (6 sigma + 4 v rmw + rho)
     cpu_gbs = bytes_cell * n_upd * n_all / s_all / 1e9
     value = n_upd * steps / secs / 1e9
@@ -392,6 +405,7 @@ def bench_2d(args) -> None:
             "achieved": round(bytes_cell * n_upd * steps / secs / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(bytes_cell * n_upd * steps / secs / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
         },
+        "parity": parity,
         "cpu_baseline": {
             "value": round(n_upd * n_all / s_all / 1e9, 5), "unit": "Gcell-updates/s", "cores": t_all, "kind": "port",
             "one_thread_value": round(n_upd * cpu[1][0] / cpu[1][1] / 1e9, 5),
