@@ -515,7 +515,10 @@ public:
     // and the constitutive law to L_e = dt M^T L_sigma.  Model gradients are
     // taken with respect to 1/rho, lambda and mu at every padded cell,
     // including the harmonic face buoyancy 2ab/(a+b) and the 4-cell harmonic
-    // shear modulus.  All-CPML boundaries only (no free surface, no walls).
+    // shear modulus.  With a free surface the ghost fills are linear maps too:
+    // their transposes move the ghost-plane adjoints back onto the cells the
+    // ghosts were built from, the velocity ghosts adding a lambda/(lambda+2mu)
+    // gradient term.  No reflecting walls.
     // ------------------------------------------------------------------
     struct Snapshot {
         std::vector<Real> f[9];
@@ -583,14 +586,91 @@ public:
         return kk[2] * le + kk[0] * total;
     }
 
+    // Transposed z-stencil at a ghost plane (z < 4) of a temporary that is zero
+    // outside the updated cells: only the taps above the plane can be nonzero.
+    template <bool plus>
+    Real ghost_derivative(const Real* field, std::size_t i, std::size_t plane, Real inverse_spacing) const {
+        Real result = 0;
+        for (int r = 0; r < 4; ++r) result += Weights<Real>::w(r) * field[i + std::size_t(r + (plus ? 1 : 0)) * plane];
+        return result * inverse_spacing;
+    }
+
+    // Adds c * d(derivative<plus>(field, i, stride))/d(field) to the adjoint `out`.
+    template <bool plus>
+    void scatter_derivative(Real* out, std::size_t i, std::size_t stride, Real c_) const {
+        for (int r = 0; r < 4; ++r) {
+            out[i + std::size_t(r + (plus ? 1 : 0)) * stride] += c_ * Weights<Real>::w(r);
+            out[i - std::size_t(r + (plus ? 0 : 1)) * stride] -= c_ * Weights<Real>::w(r);
+        }
+    }
+
+    // Transpose of stress_ghost: each ghost adjoint moves onto the cell it mirrors
+    // (with the ghost's sign) and is cleared; then the surface szz adjoint is cleared.
+    void adjoint_stress_ghost(Adjoint& A) const {
+        for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            for (int depth = 1; depth <= 4; ++depth) {
+                const auto ghost = index(x, y, 4 - depth), mirror = index(x, y, 4 + depth);
+                const auto shearMirror = index(x, y, 3 + depth);
+                A.f[5][mirror] -= A.f[5][ghost];
+                A.f[8][shearMirror] -= A.f[8][ghost];
+                A.f[7][shearMirror] -= A.f[7][ghost];
+                A.f[3][mirror] += A.f[3][ghost];
+                A.f[4][mirror] += A.f[4][ghost];
+                A.f[6][mirror] += A.f[6][ghost];
+                for (int k : {3, 4, 5, 6, 7, 8}) A.f[k][ghost] = 0;
+            }
+            A.f[5][index(x, y, 4)] = 0;
+        }
+    }
+
+    // Transpose of velocity_ghost on the velocities `v` the ghosts were built from.
+    // The ghosts are linear in the velocities and, through r = lambda/(lambda+2mu),
+    // in the model: dJ/dr = -(ex + ey) L_dzVz at each surface cell.
+    void adjoint_velocity_ghost(const Snapshot& v, Adjoint& A) const {
+        const Real invH = Real(1) / Real(c.spacing), h = Real(c.spacing);
+        const std::size_t gx = c.gx, plane = std::size_t(c.gx) * c.gy;
+        for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, 4);
+            Real l_dzvx = 0, l_dzvy = 0, l_dzvz = 0;
+            for (int depth = 1; depth <= 4; ++depth) {
+                const auto ghost = index(x, y, 4 - depth), mirror = index(x, y, 4 + depth);
+                const Real g0 = A.f[0][ghost], g1 = A.f[1][ghost], g2 = A.f[2][ghost];
+                A.f[0][mirror] += g0;
+                A.f[1][mirror] += g1;
+                A.f[2][index(x, y, 3 + depth)] += g2;
+                l_dzvx -= Real(2 * depth) * h * g0;
+                l_dzvy -= Real(2 * depth) * h * g1;
+                l_dzvz -= Real(2 * depth - 1) * h * g2;
+                A.f[0][ghost] = 0; A.f[1][ghost] = 0; A.f[2][ghost] = 0;
+            }
+            // dzVx = -1.5 D+x vz(z=4) + 0.5 D+x vz(z=5), dzVy likewise along y.
+            scatter_derivative<true>(A.f[2].data(), i, 1, Real(-1.5) * invH * l_dzvx);
+            scatter_derivative<true>(A.f[2].data(), i + plane, 1, Real(0.5) * invH * l_dzvx);
+            scatter_derivative<true>(A.f[2].data(), i, gx, Real(-1.5) * invH * l_dzvy);
+            scatter_derivative<true>(A.f[2].data(), i + plane, gx, Real(0.5) * invH * l_dzvy);
+            // dzVz = -r (ex + ey), ex = D-x vx, ey = D-y vy at the surface.
+            const Real lam = lambda[i], m = mu[i], den = lam + 2 * m;
+            const Real r = lam / den;
+            scatter_derivative<false>(A.f[0].data(), i, 1, -r * invH * l_dzvz);
+            scatter_derivative<false>(A.f[1].data(), i, gx, -r * invH * l_dzvz);
+            const Real ex = derivative<false>(v.f[0].data(), i, 1, invH);
+            const Real ey = derivative<false>(v.f[1].data(), i, gx, invH);
+            const Real gr = -(ex + ey) * l_dzvz;
+            A.g_lam[i] += gr * 2 * m / (den * den);
+            A.g_mu[i] -= gr * 2 * lam / (den * den);
+        }
+    }
+
     // Adjoint of one full step n (state `before` -> `after`), with the misfit
     // residual of a sample taken after this step already added to A.f[0..2].
     void adjoint_step(const Snapshot& before, const Snapshot& after, Adjoint& A, const seis3d_source& s,
                       std::uint64_t n) {
         const std::size_t st[3] = {1, std::size_t(c.gx), std::size_t(c.gx) * c.gy};
         const Real invH = Real(1) / Real(c.spacing), dt = Real(c.dt);
+        const bool free = c.free_surface != 0;
         std::vector<Real> ld[9];
         for (auto& v : ld) v.assign(c.cells, Real(0));
+        if (free) adjoint_stress_ghost(A);
         // ---- stress half-step ----
         for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
             const auto i = index(x, y, z);
@@ -602,16 +682,29 @@ public:
             }
             const Real lam = lambda[i], m = mu[i];
             const Real Ls[6] = {A.f[3][i], A.f[4][i], A.f[5][i], A.f[6][i], A.f[7][i], A.f[8][i]};
-            const Real sumL = Ls[0] + Ls[1] + Ls[2];
             Real le[3][3] = {};
-            for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+            if (free && z == 4) {
+                // Surface law: d_zz = -r (e_xx + e_yy) makes lambda trace = l' (e_xx + e_yy) with
+                // l' = 2 lambda mu / (lambda + 2 mu); szz is set to 0, so its adjoint clears.
+                const Real den = lam + 2 * m, lp = 2 * lam * m / den;
+                const Real sumL = Ls[0] + Ls[1], exy = e[0][0] + e[1][1];
+                le[0][0] = dt * (lp * sumL + 2 * m * Ls[0]);
+                le[1][1] = dt * (lp * sumL + 2 * m * Ls[1]);
+                const Real glp = dt * exy * sumL;
+                A.g_lam[i] += glp * 4 * m * m / (den * den);
+                A.g_mu[i] += glp * 2 * lam * lam / (den * den) + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
+                A.f[5][i] = 0;
+            } else {
+                const Real sumL = Ls[0] + Ls[1] + Ls[2];
+                for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+                const Real tr = e[0][0] + e[1][1] + e[2][2];
+                A.g_lam[i] += dt * tr * sumL;
+                A.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+            }
             const Real mxy = shear_mu(i, st[0], st[1]), myz = shear_mu(i, st[1], st[2]), mxz = shear_mu(i, st[0], st[2]);
             le[0][1] = le[1][0] = dt * mxy * Ls[3];
             le[1][2] = le[2][1] = dt * myz * Ls[4];
             le[0][2] = le[2][0] = dt * mxz * Ls[5];
-            const Real tr = e[0][0] + e[1][1] + e[2][2];
-            A.g_lam[i] += dt * tr * sumL;
-            A.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
             const Real gh[3] = {dt * (e[0][1] + e[1][0]) * Ls[3], dt * (e[1][2] + e[2][1]) * Ls[4],
                                 dt * (e[0][2] + e[2][0]) * Ls[5]};
             const std::size_t pairs[3][2] = {{st[0], st[1]}, {st[1], st[2]}, {st[0], st[2]}};
@@ -633,7 +726,15 @@ public:
                 A.f[v][i] -= t;
             }
         }
+        // The surface rows' z-stencils also read the velocity ghost planes.
+        if (free) for (int z = 0; z < 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+            const auto i = index(x, y, z);
+            for (int v = 0; v < 3; ++v)
+                A.f[v][i] -= v == 2 ? ghost_derivative<true>(ld[3 * v + 2].data(), i, st[2], invH)
+                                    : ghost_derivative<false>(ld[3 * v + 2].data(), i, st[2], invH);
+        }
         for (auto& v : ld) std::fill(v.begin(), v.end(), Real(0));
+        if (free) adjoint_velocity_ghost(after, A);
         // ---- velocity half-step ----
         const int stressFields[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
         const Real rick = ricker(c.frequency, double(n) * c.dt);
@@ -675,6 +776,16 @@ public:
                 A.f[stressFields[v][a]][i] -= t;
             }
         }
+        if (free) {
+            for (int z = 0; z < 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
+                const auto i = index(x, y, z);
+                for (int v = 0; v < 3; ++v)
+                    A.f[stressFields[v][2]][i] -= v == 2 ? ghost_derivative<false>(ld[3 * v + 2].data(), i, st[2], invH)
+                                                         : ghost_derivative<true>(ld[3 * v + 2].data(), i, st[2], invH);
+            }
+            // The step opens with stress ghosts too (a no-op on a stored state, not on an arbitrary one).
+            adjoint_stress_ghost(A);
+        }
     }
 
     // Residual injection R^T r for a sample (collocation adjoint).
@@ -696,7 +807,7 @@ public:
     // vp, vs, rho at every physical cell ([z][y][x]), storing every state.
     Real fwi_gradient(const seis3d_source& s, std::int64_t steps, int stride, const Real* observed,
                       Real* g_vp, Real* g_vs, Real* g_rho) {
-        if (c.free_surface || c.reflecting) throw Invalid("Seis3D: the adjoint supports all-CPML boundaries only.");
+        if (c.reflecting) throw Invalid("Seis3D: the adjoint does not support reflecting walls.");
         reset();
         std::vector<Snapshot> states;
         states.reserve(std::size_t(steps) + 1);

@@ -7,7 +7,8 @@ x, y. It is run block by block (x on one part of the state, y on one part), so t
 identity term can hide an error in a small off-diagonal block. Second, the model gradient
 against central differences:
 
-The adjoint has no reference counterpart; what pins it is calculus: for a misfit
+Free-surface cases add the transposes of the ghost fills and of the surface's constitutive
+law, and sample receivers on the surface. The adjoint has no reference counterpart; what pins it is calculus: for a misfit
 J(m) = 1/2 sum (gather(m) - observed)^2 the adjoint gradient g must satisfy
 <g, step> = (J(m + step) - J(m - step)) / 2 + O(|step|^3) along any direction, so the
 mismatch must fall ~9x when the step falls 3x. A dropped term, wrong sign or transposed
@@ -24,11 +25,14 @@ from cmakepro_cuda_seissim_v2_amd import workloads as W
 from oracle import pyoracle as O
 
 
-def make_case(kind, stride, nx=18, ny=16, nz=20, steps=70):
+def make_case(kind, stride, nx=18, ny=16, nz=20, steps=70, free=False):
     start, true = F.models(nx, ny, nz)
-    ctx = F.context(nx, ny, nz, true)
+    ctx = F.context(nx, ny, nz, true, free_surface=free)
     src = F.source(kind, nx, ny, nz)
     rec = F.receivers(nx, ny, nz)
+    if free:
+        # Surface receivers sample vz half in the ghost plane above the surface.
+        rec = rec + [(x, y, 0) for y in (4, ny - 5) for x in range(3, nx - 3, 4)]
     observed = gather(ctx, true, src, rec, steps, stride)
     with O.Oracle3D(ctx, 64) as o:
         o.upload_model(*start)
@@ -76,7 +80,17 @@ def force():
     return make_case(S.FORCE, stride=2)
 
 
-@pytest.mark.parametrize("name", ["explosion", "force"])
+@pytest.fixture(scope="module")
+def explosion_free():
+    return make_case(S.EXPLOSION, stride=1, free=True)
+
+
+@pytest.fixture(scope="module")
+def force_free():
+    return make_case(S.FORCE, stride=2, free=True)
+
+
+@pytest.mark.parametrize("name", ["explosion", "force", "explosion_free", "force_free"])
 def test_misfit_matches_forward(request, name):
     case = request.getfixturevalue(name)
     assert case["misfit"] > 0
@@ -86,15 +100,19 @@ def test_misfit_matches_forward(request, name):
 @pytest.mark.parametrize("name,param,seed", [
     ("explosion", 0, 11), ("explosion", 1, 12), ("explosion", 2, 13), ("explosion", 0, 14),
     ("force", 0, 21), ("force", 1, 22), ("force", 2, 23),
+    ("explosion_free", 0, 31), ("explosion_free", 1, 32), ("explosion_free", 2, 33),
+    ("force_free", 0, 41), ("force_free", 1, 42), ("force_free", 2, 43),
 ])
 def test_gradient_matches_central_differences(request, name, param, seed):
     case = request.getfixturevalue(name)
-    fd, predicted = directional(case, param, seed, 1e-3)
+    # Free-top cases show a larger third-order term at the same step (it still falls 9x per 3x,
+    # see below), so they take a 3x smaller step.
+    fd, predicted = directional(case, param, seed, 3e-4 if name.endswith("_free") else 1e-3)
     rel = abs(fd - predicted) / abs(fd)
     assert rel < 1e-4, (name, param, fd, predicted, rel)
 
 
-@pytest.mark.parametrize("name,param,seed", [("explosion", 0, 14), ("force", 2, 23)])
+@pytest.mark.parametrize("name,param,seed", [("explosion", 0, 14), ("force", 2, 23), ("force_free", 1, 42)])
 def test_mismatch_is_second_order(request, name, param, seed):
     """The FD/adjoint mismatch must be truncation error: cutting the step 3x cuts it ~9x."""
     case = request.getfixturevalue(name)
@@ -112,10 +130,13 @@ BLOCKS = ("v", "sigma", "psi_sigma", "psi_v")
 
 
 def random_state(rng, ctx, sizes, block):
-    """Random values on one block, zero elsewhere; wavefields zero in the ghost cells."""
+    """Random values on one block, zero elsewhere. Wavefields are zero in the padding, except
+    in the four ghost planes above a free surface, which the step overwrites and the adjoint
+    must clear."""
     fields = [np.zeros((ctx.gz, ctx.gy, ctx.gx)) for _ in range(9)]
     cpml = [np.zeros(n) for n in sizes]
-    inner = (slice(4, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
+    z_first = 0 if ctx.free_surface else 4
+    inner = (slice(z_first, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
     if block in ("v", "sigma"):
         for k in (range(3) if block == "v" else range(3, 9)):
             fields[k][inner] = rng.standard_normal(fields[k][inner].shape)
@@ -136,12 +157,12 @@ def restrict(ctx, sizes, fields, cpml, block):
     return np.concatenate([c[first * (n // 6):(first + 3) * (n // 6)] for c, n in zip(cpml, sizes)])
 
 
-@pytest.fixture(scope="module")
-def linear_step():
-    """An FP64 oracle on a heterogeneous all-CPML model and a zero-amplitude source."""
+@pytest.fixture(scope="module", params=["cpml", "free"])
+def linear_step(request):
+    """An FP64 oracle on a heterogeneous model (all-CPML, or a free top) and a zero-amplitude source."""
     nx = ny = nz = 16
     start, _ = F.models(nx, ny, nz)
-    ctx = F.context(nx, ny, nz, start)
+    ctx = F.context(nx, ny, nz, start, free_surface=request.param == "free")
     src = S.default_source(x=8, y=8, z=8, kind=S.FORCE, amplitude=0.0, direction=(0.6, 0.0, 0.8))
     o = O.Oracle3D(ctx, 64)
     o.upload_model(*start)
@@ -175,7 +196,7 @@ def test_adjoint_step_is_the_transpose_of_the_step(linear_step, block_in, block_
 
 
 def test_adjointness_detects_a_planted_error(linear_step):
-    """The dot-product test fails when the adjoint state is perturbed by a relative 1e-9 on one
+    """The dot-product test fails when the adjoint state is perturbed by a random relative 1e-7 on one
     block: its tolerance is far below any plausible mistake in the recursion."""
     o, ctx, src = linear_step
     sizes = [o.cpml(a).size for a in range(3)]
@@ -190,7 +211,7 @@ def test_adjointness_detects_a_planted_error(linear_step):
     sx = restrict(ctx, sizes, [o.field(k) for k in range(9)], [o.cpml(a) for a in range(3)], "sigma")
     sty_f, sty_c = o.adjoint_step(src, 3, y_f, y_c)
     sty = restrict(ctx, sizes, sty_f, sty_c, "v")
-    planted = sty * (1 + 1e-9 * rng.standard_normal(sty.shape))
+    planted = sty * (1 + 1e-7 * rng.standard_normal(sty.shape))
     lhs = float(np.dot(sx, restrict(ctx, sizes, y_f, y_c, "sigma")))
     xv = restrict(ctx, sizes, x_f, x_c, "v")
     scale = np.linalg.norm(sty) * np.linalg.norm(xv)
@@ -198,9 +219,10 @@ def test_adjointness_detects_a_planted_error(linear_step):
     assert abs(lhs - float(np.dot(xv, planted))) > 1e-12 * scale
 
 
-def test_gradient_rejects_free_surface():
+def test_gradient_rejects_reflecting_walls():
     vp, vs, rho = W.gradient_random_model(16, 16, 16, seed=3, corr=3.0, amplitude=0.05)
-    ctx = W.imported_context(16, 16, 16, max_vp=float(vp.max()), pml=8, free_surface=True, frequency=40.0)
+    ctx = W.imported_context(16, 16, 16, max_vp=float(vp.max()), pml=8, free_surface=False, frequency=40.0,
+                             reflecting=1)
     with O.Oracle3D(ctx, 64) as o:
         o.upload_model(vp, vs, rho)
         o.set_receivers([(5, 5, 0)])
