@@ -15,7 +15,8 @@
 // one).  Gradients are summed over time per cell in their cheapest form
 // (dJ/dlambda, dJ/dmu of the normal stresses, dJ/dmu_h of each harmonic shear
 // modulus, dJ/dh of each face buoyancy); the harmonic-mean derivatives are
-// applied once at the end.
+// applied once at the end.  A free surface adds the transposes of its ghost
+// fills (plane-sized passes) and of its constitutive law at z = 4.
 // Built with -ffp-contract=off, as the forward kernels.
 #include "seis3d_adjoint.hpp"
 #include "seis3d_device.hpp"
@@ -237,7 +238,6 @@ __global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
     float Ls[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) Ls[k] = P.a[3 + k][c.i];
-    const float sumL = Ls[0] + Ls[1] + Ls[2];
     const float dt = P.dt;
     const float* r = P.inv_mu;
     const long long i = c.i;
@@ -245,14 +245,30 @@ __global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
     const float myz = shear_from_inverse(r[i], r[i + st[1]], r[i + st[2]], r[i + st[1] + st[2]]);
     const float mxz = shear_from_inverse(r[i], r[i + st[0]], r[i + st[2]], r[i + st[0] + st[2]]);
     float le[3][3];
+    if (P.g.free_surface && c.z == 4) {
+        // Surface law: d_zz = -r (e_xx + e_yy) turns lambda * trace into
+        // l' (e_xx + e_yy), l' = 2 lambda mu / (lambda + 2 mu); szz is set to 0,
+        // so its adjoint clears.
+        const float den = lam + 2 * m, lp = 2 * lam * m / den;
+        const float sumL = Ls[0] + Ls[1];
+        le[0][0] = dt * (lp * sumL + 2 * m * Ls[0]);
+        le[1][1] = dt * (lp * sumL + 2 * m * Ls[1]);
+        le[2][2] = 0.0f;
+        const float glp = dt * (e[0][0] + e[1][1]) * sumL;
+        P.g_lam[i] += glp * 4 * m * m / (den * den);
+        P.g_mu[i] += glp * 2 * lam * lam / (den * den) + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
+        P.a[5][i] = 0.0f;
+    } else {
+        const float sumL = Ls[0] + Ls[1] + Ls[2];
 #pragma unroll
-    for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+        for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
+        const float tr = e[0][0] + e[1][1] + e[2][2];
+        P.g_lam[i] += dt * tr * sumL;
+        P.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+    }
     le[0][1] = le[1][0] = dt * mxy * Ls[3];
     le[1][2] = le[2][1] = dt * myz * Ls[4];
     le[0][2] = le[2][0] = dt * mxz * Ls[5];
-    const float tr = e[0][0] + e[1][1] + e[2][2];
-    P.g_lam[i] += dt * tr * sumL;
-    P.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
     P.g_shear[0][i] += dt * (e[0][1] + e[1][0]) * Ls[3];
     P.g_shear[1][i] += dt * (e[1][2] + e[2][1]) * Ls[4];
     P.g_shear[2][i] += dt * (e[0][2] + e[2][0]) * Ls[5];
@@ -338,6 +354,130 @@ __global__ __launch_bounds__(256) void adjoint_velocity_gather(AdjStep P, TileMa
 #pragma unroll
         for (int k = 0; k < 6; ++k) P.a[3 + k][c.i] = s[k];
     });
+}
+
+// ---------------------------------------------------------------------------
+// Free surface: transposes of stressGhostKernel3D and velocityGhostKernel3D
+// (cuda/Cuda_Seis3D.cu:203-219, 249-272), one thread per surface column
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool surface_column(const DevGrid& g, int& x, int& y) {
+    x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    return x < g.gx - 4 && y < g.gy - 4;
+}
+
+// Each stress ghost adjoint moves onto the cell it mirrors, with the ghost's sign,
+// and clears; then the adjoint of the surface szz (set to zero) clears.
+__global__ void adjoint_stress_ghost(AdjStep P) {
+    int x, y;
+    if (!surface_column(P.g, x, y)) return;
+    const DevGrid& g = P.g;
+#pragma unroll
+    for (int depth = 1; depth <= 4; ++depth) {
+        const long long ghost = acell(g, x, y, 4 - depth), mirror = acell(g, x, y, 4 + depth);
+        const long long shear_mirror = acell(g, x, y, 3 + depth);
+        P.a[5][mirror] -= P.a[5][ghost];
+        P.a[8][shear_mirror] -= P.a[8][ghost];
+        P.a[7][shear_mirror] -= P.a[7][ghost];
+        P.a[3][mirror] += P.a[3][ghost];
+        P.a[4][mirror] += P.a[4][ghost];
+        P.a[6][mirror] += P.a[6][ghost];
+#pragma unroll
+        for (int k = 3; k < 9; ++k) P.a[k][ghost] = 0.0f;
+    }
+    P.a[5][acell(g, x, y, 4)] = 0.0f;
+}
+
+// The z-stencils of the rows below the surface read the ghost planes.  Their
+// transposes gather the filtered adjoint derivatives (zero above z = 4) into the
+// ghost adjoints: L_v after the stress half-step (D- on v's own axis), L_sigma
+// after the velocity half-step (D+ on v's own axis).
+template <bool VELOCITY>
+__global__ void adjoint_ghost_gather(AdjStep P) {
+    int x, y;
+    if (!surface_column(P.g, x, y)) return;
+    const DevGrid& g = P.g;
+    const int zfield[3] = {5, 4, 2};   // stress index - 3 of sxz, syz, szz
+#pragma unroll
+    for (int zg = 0; zg < 4; ++zg) {
+        const long long i = acell(g, x, y, zg);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const bool plus = VELOCITY ? v != 2 : v == 2;
+            const int off = plus ? 3 : 4;
+            float s8[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int z = zg + j - off;
+                s8[j] = z >= 4 ? P.t[3 * v + 2][acell(g, x, y, z)] : 0.0f;
+            }
+            float* dst = VELOCITY ? P.a[3 + zfield[v]] : P.a[v];
+            dst[i] -= stencil8(s8, P.invh);
+        }
+    }
+}
+
+// First half of the velocity-ghost transpose, per surface column: the ghost adjoints
+// move onto the cells they copy and clear; the adjoints of the three surface slopes
+// dzVx, dzVy, dzVz (times 1/h, dzVz's times -r) go to the scratch planes; and
+// r = lambda / (lambda + 2 mu) takes its gradient dJ/dr = -(ex + ey) L_dzVz.
+__global__ void adjoint_velocity_ghost_fold(AdjStep P) {
+    int x, y;
+    if (!surface_column(P.g, x, y)) return;
+    const DevGrid& g = P.g;
+    const float h = P.spacing;
+    float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
+#pragma unroll
+    for (int depth = 1; depth <= 4; ++depth) {
+        const long long ghost = acell(g, x, y, 4 - depth), mirror = acell(g, x, y, 4 + depth);
+        const float g0 = P.a[0][ghost], g1 = P.a[1][ghost], g2 = P.a[2][ghost];
+        P.a[0][mirror] += g0;
+        P.a[1][mirror] += g1;
+        P.a[2][acell(g, x, y, 3 + depth)] += g2;
+        l0 -= float(2 * depth) * h * g0;
+        l1 -= float(2 * depth) * h * g1;
+        l2 -= float(2 * depth - 1) * h * g2;
+        P.a[0][ghost] = 0.0f; P.a[1][ghost] = 0.0f; P.a[2][ghost] = 0.0f;
+    }
+    const long long i = acell(g, x, y, 4);
+    const float lam = P.lambda[i], m = P.mu[i], den = lam + 2 * m;
+    const float ex = aderiv<false>(P.fv[0], i, 1, P.invh);
+    const float ey = aderiv<false>(P.fv[1], i, g.px, P.invh);
+    const float gr = -(ex + ey) * l2;
+    P.g_lam[i] += gr * 2 * m / (den * den);
+    P.g_mu[i] -= gr * 2 * lam / (den * den);
+    const long long col = static_cast<long long>(y) * g.px + x;
+    P.surf[0][col] = P.invh * l0;
+    P.surf[1][col] = P.invh * l1;
+    P.surf[2][col] = -(lam / den) * P.invh * l2;
+}
+
+// Second half: the transposed surface stencils of the scratch planes (zero outside
+// the surface columns).  dzVx = -1.5 D+x vz(4) + 0.5 D+x vz(5), dzVy likewise along
+// y, and dzVz = -r (D-x vx(4) + D-y vy(4)).
+__global__ void adjoint_velocity_ghost_spread(AdjStep P) {
+    int x, y;
+    if (!surface_column(P.g, x, y)) return;
+    const DevGrid& g = P.g;
+    const long long col = static_cast<long long>(y) * g.px + x, row = g.px;
+    const float* s0 = P.surf[0];
+    const float* s1 = P.surf[1];
+    const float* s2 = P.surf[2];
+    const float w[4] = {kW0, kW1, kW2, kW3};
+    float sx = 0.0f, sy = 0.0f, tx = 0.0f, ty = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        sx += w[r] * (s0[col - r - 1] - s0[col + r]);
+        sy += w[r] * (s1[col - (r + 1) * row] - s1[col + r * row]);
+        tx += w[r] * (s2[col - r] - s2[col + r + 1]);
+        ty += w[r] * (s2[col - r * row] - s2[col + (r + 1) * row]);
+    }
+    const long long i = acell(g, x, y, 4);
+    P.a[2][i] += -1.5f * sx + -1.5f * sy;
+    P.a[2][i + g.plane] += 0.5f * sx + 0.5f * sy;
+    P.a[0][i] += tx;
+    P.a[1][i] += ty;
 }
 
 // R^T r: the receiver collocation 0.5 (v[i] + v[i - e_v]) transposed.
@@ -521,16 +661,30 @@ dim3 updated_grid(const DevGrid& g) {
 hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s) {
     const TileMap m = march_tiles(P.g);
     const dim3 grid(unsigned(m.per_xcd * 8)), block(64, 4);
+    const dim3 surface(unsigned((P.g.gx - 8 + 63) / 64), unsigned((P.g.gy - 8 + 3) / 4));
+    const bool free = P.g.free_surface != 0;
+    if (free) hipLaunchKernelGGL(adjoint_stress_ghost, surface, block, 0, s, P);
     hipLaunchKernelGGL(adjoint_stress_local, updated_grid(P.g), block, 0, s, P);
     hipLaunchKernelGGL(adjoint_stress_gather, grid, block, 0, s, P, m);
+    if (free) {
+        hipLaunchKernelGGL(adjoint_ghost_gather<false>, surface, block, 0, s, P);
+        hipLaunchKernelGGL(adjoint_velocity_ghost_fold, surface, block, 0, s, P);
+        hipLaunchKernelGGL(adjoint_velocity_ghost_spread, surface, block, 0, s, P);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s) {
     const TileMap m = march_tiles(P.g);
     const dim3 grid(unsigned(m.per_xcd * 8)), block(64, 4);
+    const dim3 surface(unsigned((P.g.gx - 8 + 63) / 64), unsigned((P.g.gy - 8 + 3) / 4));
     hipLaunchKernelGGL(adjoint_velocity_local, updated_grid(P.g), block, 0, s, P);
     hipLaunchKernelGGL(adjoint_velocity_gather, grid, block, 0, s, P, m);
+    if (P.g.free_surface) {
+        hipLaunchKernelGGL(adjoint_ghost_gather<true>, surface, block, 0, s, P);
+        // The step opens with stress ghosts as well.
+        hipLaunchKernelGGL(adjoint_stress_ghost, surface, block, 0, s, P);
+    }
     return hipGetLastError();
 }
 

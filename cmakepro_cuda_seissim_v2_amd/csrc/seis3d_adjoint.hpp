@@ -28,8 +28,10 @@ struct AdjStep {
     float* g_mu;             // sum_n dJ/dmu through the normal stresses
     float* g_shear[3];       // sum_n dJ/dmu_h of the xy, yz, xz harmonic shear moduli
     float* g_buoy[3];        // sum_n dJ/dh of the x, y, z face buoyancies
+    float* surf[3];          // free surface: per-column scratch planes of the velocity-ghost transpose
     float dt;
     float invh;
+    float spacing;
     DevSource src;           // force source of step n (active only for a force source)
 };
 
@@ -57,6 +59,10 @@ struct FoldParams {
     float* g_rho;
 };
 
+// Adjoint half-steps.  With a free surface they include the transposes of the
+// ghost fills: the stress half-step opens with the stress ghosts and closes with
+// the velocity ghosts (surf holds three zeroed planes), the velocity half-step
+// closes with the stress ghosts.
 hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s);
 hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s);
 // Adds R^T r of one receiver sample r ([receiver][vx, vy, vz]) to the adjoint velocities.

@@ -11,7 +11,7 @@
 //    stresses before step s0+m and the velocities and CPML memories after it,
 //    exactly what the adjoint of that step reads;
 //  * adjoint state: 9 adjoint wavefields, adjoint CPML memories, 9 filtered
-//    derivative temporaries, 8 gradient sums.
+//    derivative temporaries, 8 gradient sums, 3 surface scratch planes.
 // Shots run on their own streams, enqueued in lockstep so that they progress
 // together; as many run at once as fit in device memory.
 #include <hip/hip_runtime.h>
@@ -37,7 +37,7 @@ struct StateView {
 
 // Floats of each part of a state, each part padded to 64 floats (256 bytes).
 struct StateLayout {
-    std::size_t cells = 0, psi[3]{};
+    std::size_t cells = 0, plane = 0, psi[3]{};
     static std::size_t pad(std::size_t n) { return align_up(n, 64); }
     std::size_t floats() const { return 9 * pad(cells) + pad(psi[0]) + pad(psi[1]) + pad(psi[2]); }
     StateView view(float* base) const {
@@ -51,6 +51,7 @@ struct StateLayout {
 StateLayout layout_for(const Plan& p) {
     StateLayout L;
     L.cells = p.cells;
+    L.plane = p.plane;
     for (int a = 0; a < 3; ++a) L.psi[a] = p.slab[a] * 6;
     return L;
 }
@@ -62,9 +63,9 @@ struct FwiPlan {
 };
 
 std::size_t adjoint_floats(const StateLayout& L) {
-    // 9 adjoint fields + adjoint psi + 9 temporaries + 8 gradient sums
-    return 26 * StateLayout::pad(L.cells) + StateLayout::pad(L.psi[0]) + StateLayout::pad(L.psi[1]) +
-           StateLayout::pad(L.psi[2]);
+    // 9 adjoint fields + adjoint psi + 9 temporaries + 8 gradient sums + 3 surface planes
+    return 26 * StateLayout::pad(L.cells) + 3 * StateLayout::pad(L.plane) + StateLayout::pad(L.psi[0]) +
+           StateLayout::pad(L.psi[1]) + StateLayout::pad(L.psi[2]);
 }
 
 FwiPlan fwi_plan(const seis3d_ctx& c, int64_t steps, int stride, int interval, int receivers) {
@@ -90,9 +91,8 @@ FwiPlan fwi_plan(const seis3d_ctx& c, int64_t steps, int stride, int interval, i
     return f;
 }
 
-void require_all_cpml(const seis3d_ctx& c) {
-    if (c.free_surface || c.reflecting)
-        invalid("Seis3D: the FWI gradient supports all-CPML boundaries only (no free surface, no reflecting walls).");
+void require_no_walls(const seis3d_ctx& c) {
+    if (c.reflecting) invalid("Seis3D: the FWI gradient does not support reflecting walls.");
 }
 
 struct ShotWork {
@@ -113,6 +113,7 @@ struct ShotWork {
     float* apsi[3]{};
     float* t[9]{};
     float* acc[8]{};                      // g_lam, g_mu, g_shear[3], g_buoy[3]
+    float* surf[3]{};                     // free-surface scratch planes
 
     void release() noexcept {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -146,6 +147,7 @@ void allocate_shot(ShotWork& w, const FwiPlan& f, const StateLayout& L, int nrec
     for (int a = 0; a < 3; ++a) w.apsi[a] = cv.floats(L.psi[a]);
     for (auto& p : w.t) p = cv.floats(L.cells);
     for (auto& p : w.acc) p = cv.floats(L.cells);
+    for (auto& p : w.surf) p = cv.floats(L.plane);
     const std::size_t record = std::size_t(f.samples) * nrec * 3;
     w.gather = cv.floats(record);
     w.observed = cv.floats(record);
@@ -198,16 +200,21 @@ struct Job {
         return P;
     }
 
-    // One all-CPML step n (runStep3D, cuda/Cuda_Seis3D.cu:602-624, without the
-    // free-surface and wall passes the gradient does not support).
+    // One step n (runStep3D, cuda/Cuda_Seis3D.cu:602-624, without the wall passes
+    // the gradient does not support).  The stress ghosts that open a step are
+    // already in place: the state starts at zero and every step ends with them.
     void forward_step(ShotWork& w, int64_t n) const {
         seis3d::DevStep P = forward_operands(w);
         seis3d::DevStep V = P;
         if (w.src.kind == SEIS3D_FORCE) V.src = make_source(c, w.src, double(n) * c.dt);
         check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
+        if (c.free_surface) check(seis3d::launch_velocity_ghost(P, c.spacing, w.stream), "kernel launch");
         seis3d::DevStep S = P;
         if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
         check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
+        // The LDS-DMA stress kernel writes the stress ghost planes itself.
+        if (c.free_surface && t->kernels.family != 4)
+            check(seis3d::launch_stress_ghost(P, w.stream), "kernel launch");
     }
 
     // Step n = s0 + m of a recomputed segment, out of place, straight into the slots:
@@ -232,6 +239,9 @@ struct Job {
         check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
         for (int k = 0; k < 3; ++k) S.p.f[k] = S.p.out[k] = cur.f[k];
         for (int k = 3; k < 9; ++k) S.p.out[k] = next.f[k];
+        // Velocity ghosts in place in slot m; the stress kernel writes the stress ghost
+        // planes into the arrays it writes.
+        if (c.free_surface) check(seis3d::launch_velocity_ghost(S, c.spacing, w.stream), "kernel launch");
         if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
         check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
     }
@@ -273,8 +283,10 @@ struct Job {
         for (int k = 0; k < 9; ++k) { A.a[k] = w.a[k]; A.t[k] = w.t[k]; }
         A.g_lam = w.acc[0]; A.g_mu = w.acc[1];
         for (int k = 0; k < 3; ++k) { A.g_shear[k] = w.acc[2 + k]; A.g_buoy[k] = w.acc[5 + k]; }
+        for (int k = 0; k < 3; ++k) A.surf[k] = w.surf[k];
         A.dt = base.dt;
         A.invh = base.invh;
+        A.spacing = c.spacing;
         if (w.src.kind == SEIS3D_FORCE) A.src = make_source(c, w.src, double(n) * c.dt);
         return A;
     }
@@ -298,9 +310,10 @@ struct Job {
         for (auto* w : shots) {
             check(hipMemsetAsync(w->flags, 0, sizeof(unsigned int), w->stream), "clear FWI flags");
             zero_state(w->stream, w->live, L);
-            // Recomputed steps never write ghost cells, so every slot must hold zero ghosts.
-            // The forward pass copies whole states into the slots of the last segment; only
-            // the slots past a short last segment start unwritten.
+            // Recomputed steps write no ghost cells but the free surface's, so every slot
+            // must hold zero ghosts elsewhere.  The forward pass copies whole states into the
+            // slots of the last segment; only the slots past a short last segment start
+            // unwritten.
             for (std::size_t m = std::size_t(N - last0); m < w->slots.size(); ++m)
                 zero_state(w->stream, w->slots[m], L);
             Batcher z{w->stream, 1};
@@ -308,6 +321,7 @@ struct Job {
             for (int a = 0; a < 3; ++a) z.add(nullptr, w->apsi[a], L.psi[a]);
             for (auto* p : w->t) z.add(nullptr, p, L.cells);
             for (auto* p : w->acc) z.add(nullptr, p, L.cells);
+            for (auto* p : w->surf) z.add(nullptr, p, L.plane);
             z.flush();
         }
         // Forward pass: receivers, checkpoints, and the last segment's slots.
@@ -383,7 +397,7 @@ int seis3d_fwi_plan(const seis3d_ctx* ctx, int64_t n_steps, int stride, int chec
         if (!ctx) invalid("Seis3D: null context.");
         seis3d_ctx c = *ctx;
         if (seis3d_validate_ctx(&c) != SEIS3D_OK) fail(SEIS3D_EINVAL, g_error);
-        require_all_cpml(c);
+        require_no_walls(c);
         const FwiPlan f = fwi_plan(c, n_steps, stride, checkpoint_interval, receivers);
         if (shot_bytes) *shot_bytes = f.shot_bytes;
         if (interval) *interval = f.interval;
@@ -395,7 +409,7 @@ int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_
     return guarded([&] {
         require_session(t);
         const seis3d_ctx& c = t->c;
-        require_all_cpml(c);
+        require_no_walls(c);
         if (!shots || count < 1 || count > 1024 || !g_vp || !g_vs || !g_rho)
             invalid("Seis3D: invalid FWI gradient request.");
         if (t->recording) fail(SEIS3D_ELIFECYCLE, "Seis3D: stop the gather before computing a gradient.");

@@ -31,8 +31,13 @@ def source(kind, nx, ny, nz, shift=0):
                             direction=(0.6, 0.0, 0.8))
 
 
-def receivers(nx, ny, nz):
-    return [(x, y, nz - 2) for y in (3, ny - 4) for x in range(2, nx - 2, 3)]
+def receivers(nx, ny, nz, surface=False):
+    """Two lines near the bottom; with `surface`, two more on the free surface, whose vz samples
+    reach half into the ghost plane above it."""
+    rec = [(x, y, nz - 2) for y in (3, ny - 4) for x in range(2, nx - 2, 3)]
+    if surface:
+        rec += [(x, y, 0) for y in (4, ny - 5) for x in range(3, nx - 3, 4)]
+    return rec
 
 
 def smooth_direction(shape, seed, base, scale):

@@ -1,6 +1,7 @@
 """FWI gradient on the GPU (seis3d_fwi_gradient): discrete adjoint with checkpoints and recompute.
 
-No reference counterpart. Pinned by
+No reference counterpart. Every case runs with all-CPML boundaries and with a free top (whose
+ghost fills and surface law have their own transposes). Pinned by
 * the FP64 and FP32 oracle adjoints on the same seeded cases (the FP64 one is itself pinned by
   central differences in test_oracle_adjoint.py);
 * central differences of the GPU's own forward misfit along seeded directions;
@@ -55,13 +56,15 @@ def gpu_misfit(ctx, model, shots, steps, stride):
     return total
 
 
-@pytest.fixture(scope="module", params=[(S.EXPLOSION, 1), (S.FORCE, 2)], ids=["explosion", "force-stride2"])
+@pytest.fixture(scope="module", params=[(S.EXPLOSION, 1, False), (S.FORCE, 2, False), (S.EXPLOSION, 1, True),
+                                        (S.FORCE, 2, True)],
+                ids=["explosion", "force-stride2", "explosion-free", "force-stride2-free"])
 def small(request):
-    kind, stride = request.param
+    kind, stride, free = request.param
     start, true = F.models(NX, NY, NZ)
-    ctx = F.context(NX, NY, NZ, true)
+    ctx = F.context(NX, NY, NZ, true, free_surface=free)
     src = F.source(kind, NX, NY, NZ)
-    rec = F.receivers(NX, NY, NZ)
+    rec = F.receivers(NX, NY, NZ, surface=free)
     obs = observed_data(ctx, true, src, rec, STEPS, stride)
     return dict(ctx=ctx, start=start, shots=[(src, rec, obs)], stride=stride)
 
@@ -146,10 +149,11 @@ def test_session_wavefield_untouched(gpu, small):
         g.run(src, 25, 5)   # the session still steps
 
 
-def test_gradient_rejects_free_surface(gpu):
+def test_gradient_rejects_reflecting_walls(gpu):
     start, true = F.models(NX, NY, NZ)
-    ctx = S.validateContext3D(S.default_context(nx=NX, ny=NY, nz=NZ, pml=8, free_surface=1, model_kind=S.IMPORTED,
-                                                max_vp=float(true[0].max()), frequency=40.0))
+    ctx = S.validateContext3D(S.default_context(nx=NX, ny=NY, nz=NZ, pml=8, free_surface=0, reflecting=1,
+                                                model_kind=S.IMPORTED, max_vp=float(true[0].max()),
+                                                frequency=40.0))
     src = F.source(S.EXPLOSION, NX, NY, NZ)
     rec = F.receivers(NX, NY, NZ)
     with S.GPUSimData3D(ctx) as g:
@@ -163,13 +167,13 @@ def test_gradient_rejects_free_surface(gpu):
 FDX, FDY, FDZ, FD_STEPS = 40, 24, 40, 160
 
 
-@pytest.fixture(scope="module")
-def medium():
+@pytest.fixture(scope="module", params=[False, True], ids=["cpml", "free"])
+def medium(request):
     if hip_device_count() < 1:
         pytest.skip("no HIP device")
     start, true = F.models(FDX, FDY, FDZ, radius=5.0)
-    ctx = F.context(FDX, FDY, FDZ, true, frequency=30.0)
-    rec = F.receivers(FDX, FDY, FDZ)
+    ctx = F.context(FDX, FDY, FDZ, true, frequency=30.0, free_surface=request.param)
+    rec = F.receivers(FDX, FDY, FDZ, surface=request.param)
     shots = []
     for k, kind in enumerate((S.EXPLOSION, S.FORCE)):
         src = F.source(kind, FDX, FDY, FDZ, shift=6 * k - 3)

@@ -29,10 +29,7 @@ def make_case(kind, stride, nx=18, ny=16, nz=20, steps=70, free=False):
     start, true = F.models(nx, ny, nz)
     ctx = F.context(nx, ny, nz, true, free_surface=free)
     src = F.source(kind, nx, ny, nz)
-    rec = F.receivers(nx, ny, nz)
-    if free:
-        # Surface receivers sample vz half in the ghost plane above the surface.
-        rec = rec + [(x, y, 0) for y in (4, ny - 5) for x in range(3, nx - 3, 4)]
+    rec = F.receivers(nx, ny, nz, surface=free)
     observed = gather(ctx, true, src, rec, steps, stride)
     with O.Oracle3D(ctx, 64) as o:
         o.upload_model(*start)
