@@ -218,7 +218,7 @@ def fwi_misfit(S, ctx, model, shots, observed, steps, stride):
 
 
 def fwi_bench(args) -> None:
-    """FWI gradient of 8 shots on 200x100x200 (1200 steps, checkpoint every 50 steps).
+    """FWI gradient of 8 shots on 200x100x200, free top (1200 steps, checkpoint every 50 steps).
 
     batched: one seis3d_fwi_gradient request with every shot (shots on their own streams, as many
     at once as fit in HBM). plain: the same library called once per shot, one after another.
@@ -300,8 +300,10 @@ def fwi_bench(args) -> None:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (smooth gradient background + seeded Gaussian anomaly, +8 % vp, radius 15 cells)",
         "config": {
-            "workload": f"FWI gradient, {len(shots)} shots, {ctx.nx}x{ctx.ny}x{ctx.nz}, dx=10 m, all-CPML 20 cells, "
-                        f"{steps} steps, 15 Hz explosions, {len(shots[0][1])} receivers stride {stride}, "
+            "workload": f"FWI gradient, {len(shots)} shots, {ctx.nx}x{ctx.ny}x{ctx.nz}, dx=10 m, "
+                        f"{'free top, CPML 20 cells on 5 faces' if ctx.free_surface else 'all-CPML 20 cells'}, "
+                        f"{steps} steps, 15 Hz explosions, {len(shots[0][1])} "
+                        f"{'surface ' if ctx.free_surface else ''}receivers stride {stride}, "
                         f"checkpoint every {interval} steps",
             "dt_us": round(ctx.dt * 1e6), "n_upd": n_upd, "shot_bytes": plan["shot_bytes"],
             "plain_shots_per_s": round(len(shots) / t_plain, 4),

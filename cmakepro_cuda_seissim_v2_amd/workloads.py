@@ -111,14 +111,15 @@ def batch_shots(count: int = 16, seed: int = BASE_SEED, sizes=None, corr: float 
 
 def fwi_workload(count: int = 8, nx: int = 200, ny: int = 100, nz: int = 200, seed: int = BASE_SEED,
                  radius: float = 15.0, gain: float = 0.08, receivers: int = 200, pml: int = 20,
-                 frequency: float = 15.0):
+                 frequency: float = 15.0, free_surface: bool = True):
     """FWI gradient workload: (context, start model, true model, [(source, receivers)] per shot).
 
     Background: smooth depth gradient vp 2000 -> 4000 m/s, vs = vp/1.73, Gardner density (the
     starting model). True model: the background with vp * (1 + gain * exp(-(r/radius)^2)) around
-    a seeded centre in the middle third of the model. All-CPML boundaries (the adjoint supports
-    no free surface). Shots: `count` 15 Hz explosions at depth 5 spread evenly along x at y = ny/2;
-    each records `receivers` receivers on a line along x at depth 2 through y = ny/2.
+    a seeded centre in the middle third of the model. A free top with CPML on the other five
+    faces (or CPML on all six). Shots: `count` 15 Hz explosions at depth 5 spread evenly along x
+    at y = ny/2; each records `receivers` receivers on a line along x through y = ny/2, on the
+    surface (depth 2 with CPML on top).
     """
     rng = np.random.default_rng(seed)
     z = np.arange(nz, dtype=np.float32) / max(1, nz - 1)
@@ -130,10 +131,11 @@ def fwi_workload(count: int = 8, nx: int = 200, ny: int = 100, nz: int = 200, se
     r2 = ((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / radius ** 2
     vp_true = (vp * (1.0 + gain * np.exp(-r2))).astype(np.float32)
     true = (vp_true, vs, gardner_density(vp))
-    ctx = imported_context(nx, ny, nz, max_vp=float(vp_true.max()), pml=pml, free_surface=False,
+    ctx = imported_context(nx, ny, nz, max_vp=float(vp_true.max()), pml=pml, free_surface=free_surface,
                            frequency=frequency)
     nrec = min(receivers, nx)
-    rec = [(int(round(i * (nx - 1) / max(1, nrec - 1))), ny // 2, 2) for i in range(nrec)]
+    depth = 0 if free_surface else 2
+    rec = [(int(round(i * (nx - 1) / max(1, nrec - 1))), ny // 2, depth) for i in range(nrec)]
     shots = []
     for k in range(count):
         sx = int(round(nx * (k + 0.5) / count))
