@@ -16,7 +16,8 @@
 // (dJ/dlambda, dJ/dmu of the normal stresses, dJ/dmu_h of each harmonic shear
 // modulus, dJ/dh of each face buoyancy); the harmonic-mean derivatives are
 // applied once at the end.  A free surface adds the transposes of its ghost
-// fills (plane-sized passes) and of its constitutive law at z = 4.
+// fills (plane-sized passes) and of its constitutive law at z = 4; reflecting
+// walls add the transposes of their mirror fills.
 // Built with -ffp-contract=off, as the forward kernels.
 #include "seis3d_adjoint.hpp"
 #include "seis3d_device.hpp"
@@ -480,6 +481,80 @@ __global__ void adjoint_velocity_ghost_spread(AdjStep P) {
     P.a[1][i] += ty;
 }
 
+// ---------------------------------------------------------------------------
+// Reflecting walls: transpose of reflectingGhostKernel3D (cuda/Cuda_Seis3D.cu:152-175)
+// ---------------------------------------------------------------------------
+
+// One axis per launch, axes in reverse order of the forward's.  On each line along
+// the axis the zeroed cells' adjoints clear and each ghost adjoint moves onto its
+// source, with the ghost's sign, and clears.
+__global__ void adjoint_reflecting_walls(AdjStep P, int axis, int velocities) {
+    const DevGrid& g = P.g;
+    const int lengths[3] = {g.gx, g.gy, g.gz};
+    const int u_axis = (axis + 1) % 3, v_axis = (axis + 2) % 3;
+    const int u = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int v = int(blockIdx.y * blockDim.y + threadIdx.y);
+    if (u >= lengths[u_axis] || v >= lengths[v_axis]) return;
+    const int high = lengths[axis] - 4 - 1;
+    int p[3] = {0, 0, 0};
+    p[u_axis] = u; p[v_axis] = v;
+    for (int field = velocities ? 0 : 3; field < (velocities ? 3 : 9); ++field) {
+        const bool half = field < 3 ? field == axis : field == 6 ? axis != 2 : field == 7 ? axis != 0
+                        : field == 8 ? axis != 1 : false;
+        float* L = P.a[field];
+        if (half) { p[axis] = high; L[acell(g, p[0], p[1], p[2])] = 0.0f; }
+        for (int side = 1; side >= 0; --side) for (int depth = 4; depth >= 1; --depth) {
+            p[axis] = side ? high + depth : 4 - depth;
+            const long long destination = acell(g, p[0], p[1], p[2]);
+            if (!(half && !side && depth == 1)) {
+                p[axis] = side ? high + 1 - depth - int(half) : 4 + depth - 1 - int(half);
+                const long long source = acell(g, p[0], p[1], p[2]);
+                L[source] += half ? -L[destination] : L[destination];
+            }
+            L[destination] = 0.0f;
+        }
+    }
+}
+
+// With reflecting walls every face's ghost cells are read by the stencils of the
+// cells next to it.  One launch per axis: threads over the face's updated (u, v)
+// cells, both sides and all four depths, gathering the normal-axis stencil of the
+// temporaries (zero outside the updated cells) into the ghost adjoints.
+template <bool VELOCITY>
+__global__ void adjoint_face_gather(AdjStep P, int axis) {
+    const DevGrid& g = P.g;
+    const int lengths[3] = {g.gx, g.gy, g.gz};
+    const int u_axis = (axis + 1) % 3, v_axis = (axis + 2) % 3;
+    const int u = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int v = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    if (u >= lengths[u_axis] - 4 || v >= lengths[v_axis] - 4) return;
+    const int n = lengths[axis];
+    const long long stride = axis == 0 ? 1 : axis == 1 ? g.px : g.plane;
+    const int zfield[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};   // stress index - 3
+    int p[3] = {0, 0, 0};
+    p[u_axis] = u; p[v_axis] = v;
+    for (int side = 0; side < 2; ++side)
+        for (int depth = 0; depth < 4; ++depth) {
+            const int coord = side ? n - 4 + depth : depth;
+            p[axis] = coord;
+            const long long i = acell(g, p[0], p[1], p[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bool plus = VELOCITY ? axis != c : axis == c;
+                const int off = plus ? 3 : 4;
+                const float* t = P.t[3 * c + axis];
+                float s8[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = coord + j - off;
+                    s8[j] = k >= 4 && k < n - 4 ? t[i + (j - off) * stride] : 0.0f;
+                }
+                float* dst = VELOCITY ? P.a[3 + zfield[c][axis]] : P.a[c];
+                dst[i] -= stencil8(s8, P.invh);
+            }
+        }
+}
+
 // R^T r: the receiver collocation 0.5 (v[i] + v[i - e_v]) transposed.
 __global__ void adjoint_inject(AdjStep P, const int* __restrict__ rec, int count, const float* __restrict__ res) {
     const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -651,6 +726,26 @@ TileMap march_tiles(const DevGrid& g) {
     return m;
 }
 
+// Transposed walls of one field set: axes 2, 1, 0 (the forward runs 0, 1, 2).
+void launch_walls_adjoint(const AdjStep& P, bool velocities, hipStream_t s) {
+    const int lengths[3] = {P.g.gx, P.g.gy, P.g.gz};
+    const dim3 block(16, 16);
+    for (int axis = 2; axis >= 0; --axis) {
+        const dim3 grid((lengths[(axis + 1) % 3] + 15) / 16, (lengths[(axis + 2) % 3] + 15) / 16);
+        hipLaunchKernelGGL(adjoint_reflecting_walls, grid, block, 0, s, P, axis, velocities ? 1 : 0);
+    }
+}
+
+template <bool VELOCITY>
+void launch_face_gathers(const AdjStep& P, hipStream_t s) {
+    const int lengths[3] = {P.g.gx, P.g.gy, P.g.gz};
+    const dim3 block(16, 16);
+    for (int axis = 0; axis < 3; ++axis) {
+        const dim3 grid((lengths[(axis + 1) % 3] - 8 + 15) / 16, (lengths[(axis + 2) % 3] - 8 + 15) / 16);
+        hipLaunchKernelGGL(adjoint_face_gather<VELOCITY>, grid, block, 0, s, P, axis);
+    }
+}
+
 // One thread per updated cell: a 64 x 4 tile of one plane per workgroup.
 dim3 updated_grid(const DevGrid& g) {
     return dim3(unsigned((g.gx - 8 + 63) / 64), unsigned((g.gy - 8 + 3) / 4), unsigned(g.gz - 8));
@@ -664,12 +759,17 @@ hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s) {
     const dim3 surface(unsigned((P.g.gx - 8 + 63) / 64), unsigned((P.g.gy - 8 + 3) / 4));
     const bool free = P.g.free_surface != 0;
     if (free) hipLaunchKernelGGL(adjoint_stress_ghost, surface, block, 0, s, P);
+    if (P.reflecting) launch_walls_adjoint(P, false, s);
     hipLaunchKernelGGL(adjoint_stress_local, updated_grid(P.g), block, 0, s, P);
     hipLaunchKernelGGL(adjoint_stress_gather, grid, block, 0, s, P, m);
     if (free) {
         hipLaunchKernelGGL(adjoint_ghost_gather<false>, surface, block, 0, s, P);
         hipLaunchKernelGGL(adjoint_velocity_ghost_fold, surface, block, 0, s, P);
         hipLaunchKernelGGL(adjoint_velocity_ghost_spread, surface, block, 0, s, P);
+    }
+    if (P.reflecting) {
+        launch_face_gathers<false>(P, s);
+        launch_walls_adjoint(P, true, s);
     }
     return hipGetLastError();
 }
@@ -684,6 +784,11 @@ hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s) {
         hipLaunchKernelGGL(adjoint_ghost_gather<true>, surface, block, 0, s, P);
         // The step opens with stress ghosts as well.
         hipLaunchKernelGGL(adjoint_stress_ghost, surface, block, 0, s, P);
+    }
+    if (P.reflecting) {
+        launch_face_gathers<true>(P, s);
+        // The step opens with stress walls as well.
+        launch_walls_adjoint(P, false, s);
     }
     return hipGetLastError();
 }

@@ -32,6 +32,7 @@ struct AdjStep {
     float dt;
     float invh;
     float spacing;
+    int reflecting;          // reflecting walls (then the grid has no CPML)
     DevSource src;           // force source of step n (active only for a force source)
 };
 
@@ -59,10 +60,10 @@ struct FoldParams {
     float* g_rho;
 };
 
-// Adjoint half-steps.  With a free surface they include the transposes of the
-// ghost fills: the stress half-step opens with the stress ghosts and closes with
-// the velocity ghosts (surf holds three zeroed planes), the velocity half-step
-// closes with the stress ghosts.
+// Adjoint half-steps.  With a free surface or reflecting walls they include the
+// transposes of the ghost fills: the stress half-step opens with the stress
+// ghosts and closes with the velocity ghosts (surf holds three zeroed planes),
+// the velocity half-step closes with the stress ghosts.
 hipError_t launch_adjoint_stress(const AdjStep& P, hipStream_t s);
 hipError_t launch_adjoint_velocity(const AdjStep& P, hipStream_t s);
 // Adds R^T r of one receiver sample r ([receiver][vx, vy, vz]) to the adjoint velocities.

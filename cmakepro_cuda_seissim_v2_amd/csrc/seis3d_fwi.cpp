@@ -91,10 +91,6 @@ FwiPlan fwi_plan(const seis3d_ctx& c, int64_t steps, int stride, int interval, i
     return f;
 }
 
-void require_no_walls(const seis3d_ctx& c) {
-    if (c.reflecting) invalid("Seis3D: the FWI gradient does not support reflecting walls.");
-}
-
 struct ShotWork {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -200,21 +196,23 @@ struct Job {
         return P;
     }
 
-    // One step n (runStep3D, cuda/Cuda_Seis3D.cu:602-624, without the wall passes
-    // the gradient does not support).  The stress ghosts that open a step are
-    // already in place: the state starts at zero and every step ends with them.
+    // One step n (runStep3D, cuda/Cuda_Seis3D.cu:602-624).  The stress ghosts and
+    // walls that open a step are already in place: the state starts at zero and
+    // every step ends with them.
     void forward_step(ShotWork& w, int64_t n) const {
         seis3d::DevStep P = forward_operands(w);
         seis3d::DevStep V = P;
         if (w.src.kind == SEIS3D_FORCE) V.src = make_source(c, w.src, double(n) * c.dt);
         check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
         if (c.free_surface) check(seis3d::launch_velocity_ghost(P, c.spacing, w.stream), "kernel launch");
+        if (c.reflecting) check(seis3d::launch_reflecting_walls(P, true, w.stream), "kernel launch");
         seis3d::DevStep S = P;
         if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
         check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
         // The LDS-DMA stress kernel writes the stress ghost planes itself.
         if (c.free_surface && t->kernels.family != 4)
             check(seis3d::launch_stress_ghost(P, w.stream), "kernel launch");
+        if (c.reflecting) check(seis3d::launch_reflecting_walls(P, false, w.stream), "kernel launch");
     }
 
     // Step n = s0 + m of a recomputed segment, out of place, straight into the slots:
@@ -239,11 +237,17 @@ struct Job {
         check(seis3d::launch_velocity(V, t->kernels, w.stream), "kernel launch");
         for (int k = 0; k < 3; ++k) S.p.f[k] = S.p.out[k] = cur.f[k];
         for (int k = 3; k < 9; ++k) S.p.out[k] = next.f[k];
-        // Velocity ghosts in place in slot m; the stress kernel writes the stress ghost
-        // planes into the arrays it writes.
+        // Velocity ghosts and walls in place in slot m; the stress kernel writes the
+        // stress ghost planes into the arrays it writes, and the stress walls follow there.
         if (c.free_surface) check(seis3d::launch_velocity_ghost(S, c.spacing, w.stream), "kernel launch");
+        if (c.reflecting) check(seis3d::launch_reflecting_walls(S, true, w.stream), "kernel launch");
         if (w.src.kind == SEIS3D_EXPLOSION) S.src = make_source(c, w.src, (double(n) + 0.5) * c.dt);
         check(seis3d::launch_stress(S, t->kernels, w.stream), "kernel launch");
+        if (c.reflecting) {
+            seis3d::DevStep W = S;
+            for (int k = 3; k < 9; ++k) W.p.f[k] = W.p.out[k] = next.f[k];
+            check(seis3d::launch_reflecting_walls(W, false, w.stream), "kernel launch");
+        }
     }
 
     // After step n inside the segment starting at s0: velocities and psi into
@@ -287,6 +291,7 @@ struct Job {
         A.dt = base.dt;
         A.invh = base.invh;
         A.spacing = c.spacing;
+        A.reflecting = c.reflecting;
         if (w.src.kind == SEIS3D_FORCE) A.src = make_source(c, w.src, double(n) * c.dt);
         return A;
     }
@@ -310,8 +315,8 @@ struct Job {
         for (auto* w : shots) {
             check(hipMemsetAsync(w->flags, 0, sizeof(unsigned int), w->stream), "clear FWI flags");
             zero_state(w->stream, w->live, L);
-            // Recomputed steps write no ghost cells but the free surface's, so every slot
-            // must hold zero ghosts elsewhere.  The forward pass copies whole states into the
+            // Recomputed steps write no ghost cells but the free surface's and the walls',
+            // so every slot must hold zero ghosts elsewhere.  The forward pass copies whole states into the
             // slots of the last segment; only the slots past a short last segment start
             // unwritten.
             for (std::size_t m = std::size_t(N - last0); m < w->slots.size(); ++m)
@@ -397,7 +402,6 @@ int seis3d_fwi_plan(const seis3d_ctx* ctx, int64_t n_steps, int stride, int chec
         if (!ctx) invalid("Seis3D: null context.");
         seis3d_ctx c = *ctx;
         if (seis3d_validate_ctx(&c) != SEIS3D_OK) fail(SEIS3D_EINVAL, g_error);
-        require_no_walls(c);
         const FwiPlan f = fwi_plan(c, n_steps, stride, checkpoint_interval, receivers);
         if (shot_bytes) *shot_bytes = f.shot_bytes;
         if (interval) *interval = f.interval;
@@ -409,7 +413,6 @@ int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_
     return guarded([&] {
         require_session(t);
         const seis3d_ctx& c = t->c;
-        require_no_walls(c);
         if (!shots || count < 1 || count > 1024 || !g_vp || !g_vs || !g_rho)
             invalid("Seis3D: invalid FWI gradient request.");
         if (t->recording) fail(SEIS3D_ELIFECYCLE, "Seis3D: stop the gather before computing a gradient.");

@@ -593,7 +593,7 @@ class GPUSimData3D:
         """Misfit and gradient ([nz, ny, nx] float32 vp, vs, rho) of shots (source, receivers, observed).
 
         `observed` of each shot is [n_steps // stride, len(receivers), 3]; shots share this session's
-        model; CPML and free-surface contexts (no reflecting walls).
+        model; any boundary (CPML, a free top, reflecting walls).
         """
         c = self._ctx_now()
         keep = []

@@ -298,7 +298,7 @@ int seis3d_fwi_plan(const seis3d_ctx* ctx, int64_t n_steps, int stride, int chec
  * checkpoint_interval steps and recomputed segment by segment.  Shots share
  * the session's model and run concurrently as device memory allows; the
  * session's own wavefield, receivers and gather are left untouched.
- * CPML and free-surface contexts; reflecting walls give EINVAL. */
+ * Any boundary: CPML, a free top, or reflecting walls. */
 int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_t n_steps, int stride,
                         int checkpoint_interval, double* misfit, float* g_vp, float* g_vs, float* g_rho);
 

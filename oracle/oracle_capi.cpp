@@ -220,11 +220,10 @@ int oracle3d_set_cpml(Handle* h, int axis, const double* in) {
 // Adjoint of one step n applied in place to an adjoint state laid out as the nine
 // wavefields over the padded grid followed by the three CPML memory arrays.  The
 // solver's own state stands in for the stored forward states, which only enter the
-// model-gradient sums (discarded here).  No reflecting walls.
+// model-gradient sums (discarded here).
 int oracle3d_adjoint_step(Handle* h, const seis3d_source* src, uint64_t step, double* state) {
     return guard([&] {
         with_solver(h, [&](auto& s) {
-            if (s.c.reflecting) throw std::invalid_argument("Seis3D: the adjoint does not support reflecting walls.");
             auto A = s.make_adjoint();
             double* p = state;
             for (auto& v : A.f) for (auto& x : v) x = *p++;

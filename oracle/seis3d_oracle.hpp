@@ -515,10 +515,10 @@ public:
     // and the constitutive law to L_e = dt M^T L_sigma.  Model gradients are
     // taken with respect to 1/rho, lambda and mu at every padded cell,
     // including the harmonic face buoyancy 2ab/(a+b) and the 4-cell harmonic
-    // shear modulus.  With a free surface the ghost fills are linear maps too:
-    // their transposes move the ghost-plane adjoints back onto the cells the
-    // ghosts were built from, the velocity ghosts adding a lambda/(lambda+2mu)
-    // gradient term.  No reflecting walls.
+    // shear modulus.  The free surface's ghost fills and the reflecting walls are
+    // linear maps too: their transposes move the ghost adjoints back onto the
+    // cells the ghosts were built from, the velocity ghosts of a free surface
+    // adding a lambda/(lambda+2mu) gradient term.
     // ------------------------------------------------------------------
     struct Snapshot {
         std::vector<Real> f[9];
@@ -586,13 +586,80 @@ public:
         return kk[2] * le + kk[0] * total;
     }
 
-    // Transposed z-stencil at a ghost plane (z < 4) of a temporary that is zero
-    // outside the updated cells: only the taps above the plane can be nonzero.
+    // Stencil along an axis at a ghost cell (coordinate `coord` of `n`) of a temporary
+    // that is zero outside the updated cells: taps outside [4, n - 4) are zero, and
+    // are not read (they may lie outside the grid).
     template <bool plus>
-    Real ghost_derivative(const Real* field, std::size_t i, std::size_t plane, Real inverse_spacing) const {
+    Real clipped_derivative(const Real* field, std::size_t i, int coord, int n, std::size_t stride,
+                            Real inverse_spacing) const {
         Real result = 0;
-        for (int r = 0; r < 4; ++r) result += Weights<Real>::w(r) * field[i + std::size_t(r + (plus ? 1 : 0)) * plane];
+        for (int r = 0; r < 4; ++r) {
+            const int up = r + (plus ? 1 : 0), down = r + (plus ? 0 : 1);
+            const Real fp = coord + up >= 4 && coord + up < n - 4 ? field[i + std::size_t(up) * stride] : Real(0);
+            const Real fn = coord - down >= 4 && coord - down < n - 4 ? field[i - std::size_t(down) * stride] : Real(0);
+            result += Weights<Real>::w(r) * (fp - fn);
+        }
         return result * inverse_spacing;
+    }
+
+    // The stencils of the cells next to a boundary read its ghost cells: the four
+    // planes above a free surface, or every face's with reflecting walls.  The
+    // transposed stencils gather the temporaries L_d into those ghost adjoints (only
+    // the stencil along the face normal reaches them): L_v after the stress half-step,
+    // L_sigma after the velocity half-step.
+    void ghost_gathers(const std::vector<Real> (&ld)[9], Adjoint& A, bool velocity_pass) const {
+        const int stressFields[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
+        const int n[3] = {c.gx, c.gy, c.gz};
+        const std::size_t st[3] = {1, std::size_t(c.gx), std::size_t(c.gx) * c.gy};
+        const Real invH = Real(1) / Real(c.spacing);
+        for (int axis = 0; axis < 3; ++axis) for (int side = 0; side < 2; ++side) {
+            if (!c.reflecting && !(c.free_surface && axis == 2 && side == 0)) continue;
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) { lo[a] = 4; hi[a] = n[a] - 4; }
+            lo[axis] = side ? n[axis] - 4 : 0;
+            hi[axis] = side ? n[axis] : 4;
+            for (int z = lo[2]; z < hi[2]; ++z) for (int y = lo[1]; y < hi[1]; ++y) for (int x = lo[0]; x < hi[0]; ++x) {
+                const auto i = index(x, y, z);
+                const int coord = axis == 0 ? x : axis == 1 ? y : z;
+                for (int v = 0; v < 3; ++v) {
+                    const bool plus = velocity_pass ? axis != v : axis == v;
+                    const Real* t = ld[3 * v + axis].data();
+                    const Real d = plus ? clipped_derivative<true>(t, i, coord, n[axis], st[axis], invH)
+                                        : clipped_derivative<false>(t, i, coord, n[axis], st[axis], invH);
+                    A.f[velocity_pass ? stressFields[v][axis] : v][i] -= d;
+                }
+            }
+        }
+    }
+
+    // Transpose of reflecting_walls: axes in reverse order; on each line the zeroed
+    // cells' adjoints clear and each ghost adjoint moves onto its source (with the
+    // ghost's sign) and clears.
+    void adjoint_reflecting_walls(Adjoint& A, bool velocities) const {
+        const int lengths[3] = {c.gx, c.gy, c.gz};
+        for (int axis = 2; axis >= 0; --axis) {
+            const int uAxis = (axis + 1) % 3, vAxis = (axis + 2) % 3;
+            const int high = lengths[axis] - kRadius - 1;
+            for (int v = 0; v < lengths[vAxis]; ++v) for (int u = 0; u < lengths[uAxis]; ++u) {
+                int p[3]{}; p[uAxis] = u; p[vAxis] = v;
+                for (int field = velocities ? 0 : 3; field < (velocities ? 3 : 9); ++field) {
+                    const bool half = field < 3 ? field == axis : field == 6 ? axis != 2 : field == 7 ? axis != 0
+                        : field == 8 ? axis != 1 : false;
+                    Real* L = A.f[field].data();
+                    if (half) { p[axis] = high; L[index(p[0], p[1], p[2])] = 0; }
+                    for (int side = 1; side >= 0; --side) for (int depth = kRadius; depth >= 1; --depth) {
+                        p[axis] = side ? high + depth : kRadius - depth;
+                        const auto destination = index(p[0], p[1], p[2]);
+                        if (!(half && !side && depth == 1)) {
+                            p[axis] = side ? high + 1 - depth - int(half) : kRadius + depth - 1 - int(half);
+                            const auto source = index(p[0], p[1], p[2]);
+                            L[source] += half ? -L[destination] : L[destination];
+                        }
+                        L[destination] = 0;
+                    }
+                }
+            }
+        }
     }
 
     // Adds c * d(derivative<plus>(field, i, stride))/d(field) to the adjoint `out`.
@@ -667,10 +734,11 @@ public:
                       std::uint64_t n) {
         const std::size_t st[3] = {1, std::size_t(c.gx), std::size_t(c.gx) * c.gy};
         const Real invH = Real(1) / Real(c.spacing), dt = Real(c.dt);
-        const bool free = c.free_surface != 0;
+        const bool free = c.free_surface != 0, walls = c.reflecting != 0;
         std::vector<Real> ld[9];
         for (auto& v : ld) v.assign(c.cells, Real(0));
         if (free) adjoint_stress_ghost(A);
+        if (walls) adjoint_reflecting_walls(A, false);
         // ---- stress half-step ----
         for (int z = 4; z < c.gz - 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
             const auto i = index(x, y, z);
@@ -726,15 +794,10 @@ public:
                 A.f[v][i] -= t;
             }
         }
-        // The surface rows' z-stencils also read the velocity ghost planes.
-        if (free) for (int z = 0; z < 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
-            const auto i = index(x, y, z);
-            for (int v = 0; v < 3; ++v)
-                A.f[v][i] -= v == 2 ? ghost_derivative<true>(ld[3 * v + 2].data(), i, st[2], invH)
-                                    : ghost_derivative<false>(ld[3 * v + 2].data(), i, st[2], invH);
-        }
+        ghost_gathers(ld, A, false);
         for (auto& v : ld) std::fill(v.begin(), v.end(), Real(0));
         if (free) adjoint_velocity_ghost(after, A);
+        if (walls) adjoint_reflecting_walls(A, true);
         // ---- velocity half-step ----
         const int stressFields[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
         const Real rick = ricker(c.frequency, double(n) * c.dt);
@@ -776,16 +839,11 @@ public:
                 A.f[stressFields[v][a]][i] -= t;
             }
         }
-        if (free) {
-            for (int z = 0; z < 4; ++z) for (int y = 4; y < c.gy - 4; ++y) for (int x = 4; x < c.gx - 4; ++x) {
-                const auto i = index(x, y, z);
-                for (int v = 0; v < 3; ++v)
-                    A.f[stressFields[v][2]][i] -= v == 2 ? ghost_derivative<false>(ld[3 * v + 2].data(), i, st[2], invH)
-                                                         : ghost_derivative<true>(ld[3 * v + 2].data(), i, st[2], invH);
-            }
-            // The step opens with stress ghosts too (a no-op on a stored state, not on an arbitrary one).
-            adjoint_stress_ghost(A);
-        }
+        ghost_gathers(ld, A, true);
+        // The step opens with stress ghosts and walls too (no-ops on a stored state, not
+        // on an arbitrary one).
+        if (free) adjoint_stress_ghost(A);
+        if (walls) adjoint_reflecting_walls(A, false);
     }
 
     // Residual injection R^T r for a sample (collocation adjoint).
@@ -807,7 +865,6 @@ public:
     // vp, vs, rho at every physical cell ([z][y][x]), storing every state.
     Real fwi_gradient(const seis3d_source& s, std::int64_t steps, int stride, const Real* observed,
                       Real* g_vp, Real* g_vs, Real* g_rho) {
-        if (c.reflecting) throw Invalid("Seis3D: the adjoint does not support reflecting walls.");
         reset();
         std::vector<Snapshot> states;
         states.reserve(std::size_t(steps) + 1);

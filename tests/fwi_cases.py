@@ -2,7 +2,7 @@
 
 A smooth gradient model with a weak random field is the starting model; the true model adds a
 Gaussian anomaly (+8 % vp, -4 % rho) in the middle. All-CPML boundaries unless a case asks for a
-free top; receivers on two lines near the bottom, the source near the top.
+free top or reflecting walls; receivers on two lines near the bottom, the source near the top.
 """
 import numpy as np
 
@@ -19,9 +19,10 @@ def models(nx, ny, nz, seed=2, radius=3.0, vp_gain=0.08, rho_gain=-0.04):
     return (vp, vs, rho), (vp_true, vs, rho_true)
 
 
-def context(nx, ny, nz, true_model, pml=8, frequency=40.0, free_surface=False):
+def context(nx, ny, nz, true_model, pml=8, frequency=40.0, free_surface=False, reflecting=False):
     return W.imported_context(nx, ny, nz, max_vp=float(true_model[0].max() * 1.05), pml=pml,
-                              free_surface=free_surface, frequency=frequency)
+                              free_surface=free_surface and not reflecting, frequency=frequency,
+                              reflecting=int(reflecting))
 
 
 def source(kind, nx, ny, nz, shift=0):

@@ -136,11 +136,9 @@ def test_fwi_plan_memory_formula():
 
 
 def test_fwi_plan_rejects_unsupported_requests():
-    walls = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=False, reflecting=1)
-    with pytest.raises(S.InvalidArgument, match="reflecting"):
-        S.fwi_plan(walls, 100, 1, 10, 5)
-    free = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=True)
-    assert S.fwi_plan(free, 100, 1, 10, 5)["interval"] == 10
+    for free, walls in ((True, 0), (False, 1)):
+        ok = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=free, reflecting=walls)
+        assert S.fwi_plan(ok, 100, 1, 10, 5)["interval"] == 10
     c = W.imported_context(32, 32, 32, max_vp=4000.0, pml=8, free_surface=False)
     for steps, stride, rec in ((0, 1, 5), (100, 0, 5), (100, 101, 5), (100, 1, 0), (100, 1, 301)):
         with pytest.raises(S.InvalidArgument):

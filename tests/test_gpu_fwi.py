@@ -1,7 +1,7 @@
 """FWI gradient on the GPU (seis3d_fwi_gradient): discrete adjoint with checkpoints and recompute.
 
-No reference counterpart. Every case runs with all-CPML boundaries and with a free top (whose
-ghost fills and surface law have their own transposes). Pinned by
+No reference counterpart. Every case runs with all-CPML boundaries, with a free top (whose ghost
+fills and surface law have their own transposes) and with reflecting walls. Pinned by
 * the FP64 and FP32 oracle adjoints on the same seeded cases (the FP64 one is itself pinned by
   central differences in test_oracle_adjoint.py);
 * central differences of the GPU's own forward misfit along seeded directions;
@@ -56,15 +56,16 @@ def gpu_misfit(ctx, model, shots, steps, stride):
     return total
 
 
-@pytest.fixture(scope="module", params=[(S.EXPLOSION, 1, False), (S.FORCE, 2, False), (S.EXPLOSION, 1, True),
-                                        (S.FORCE, 2, True)],
-                ids=["explosion", "force-stride2", "explosion-free", "force-stride2-free"])
+@pytest.fixture(scope="module", params=[(S.EXPLOSION, 1, "cpml"), (S.FORCE, 2, "cpml"), (S.EXPLOSION, 1, "free"),
+                                        (S.FORCE, 2, "free"), (S.EXPLOSION, 1, "walls"), (S.FORCE, 2, "walls")],
+                ids=["explosion", "force-stride2", "explosion-free", "force-stride2-free", "explosion-walls",
+                     "force-stride2-walls"])
 def small(request):
-    kind, stride, free = request.param
+    kind, stride, boundary = request.param
     start, true = F.models(NX, NY, NZ)
-    ctx = F.context(NX, NY, NZ, true, free_surface=free)
+    ctx = F.context(NX, NY, NZ, true, free_surface=boundary == "free", reflecting=boundary == "walls")
     src = F.source(kind, NX, NY, NZ)
-    rec = F.receivers(NX, NY, NZ, surface=free)
+    rec = F.receivers(NX, NY, NZ, surface=boundary == "free")
     obs = observed_data(ctx, true, src, rec, STEPS, stride)
     return dict(ctx=ctx, start=start, shots=[(src, rec, obs)], stride=stride)
 
@@ -149,31 +150,19 @@ def test_session_wavefield_untouched(gpu, small):
         g.run(src, 25, 5)   # the session still steps
 
 
-def test_gradient_rejects_reflecting_walls(gpu):
-    start, true = F.models(NX, NY, NZ)
-    ctx = S.validateContext3D(S.default_context(nx=NX, ny=NY, nz=NZ, pml=8, free_surface=0, reflecting=1,
-                                                model_kind=S.IMPORTED, max_vp=float(true[0].max()),
-                                                frequency=40.0))
-    src = F.source(S.EXPLOSION, NX, NY, NZ)
-    rec = F.receivers(NX, NY, NZ)
-    with S.GPUSimData3D(ctx) as g:
-        g.upload_model(*start)
-        with pytest.raises(S.InvalidArgument):
-            g.fwi_gradient([(src, rec, np.zeros((10, len(rec), 3), np.float32))], 10, 1)
-
-
 # ---- central differences of the GPU forward misfit -----------------------------------------------
 
 FDX, FDY, FDZ, FD_STEPS = 40, 24, 40, 160
 
 
-@pytest.fixture(scope="module", params=[False, True], ids=["cpml", "free"])
+@pytest.fixture(scope="module", params=["cpml", "free", "walls"])
 def medium(request):
     if hip_device_count() < 1:
         pytest.skip("no HIP device")
     start, true = F.models(FDX, FDY, FDZ, radius=5.0)
-    ctx = F.context(FDX, FDY, FDZ, true, frequency=30.0, free_surface=request.param)
-    rec = F.receivers(FDX, FDY, FDZ, surface=request.param)
+    ctx = F.context(FDX, FDY, FDZ, true, frequency=30.0, free_surface=request.param == "free",
+                    reflecting=request.param == "walls")
+    rec = F.receivers(FDX, FDY, FDZ, surface=request.param == "free")
     shots = []
     for k, kind in enumerate((S.EXPLOSION, S.FORCE)):
         src = F.source(kind, FDX, FDY, FDZ, shift=6 * k - 3)
