@@ -308,6 +308,24 @@ def test_p_wave_moveout_matches_vp():
     assert lag == pytest.approx(expected, rel=0.02), (lag, expected)
 
 
+def test_s_wave_moveout_matches_vs():
+    """Lag of the direct S arrival between two receivers on a ray equals distance / vs: a force
+    across the ray radiates no far-field P along it, so the transverse component carries the S
+    wave alone."""
+    c = uniform_ctx(n=56, pml=10, frequency=40.0)
+    src = S.default_source(x=10, y=28, z=28, kind=S.FORCE, direction=(0.0, 1.0, 0.0))
+    rec = [(22, 28, 28), (40, 28, 28)]
+    _, g = run_gather(c, 32, 420, src, rec)
+    a, b = g[:, 0, 1].astype(np.float64), g[:, 1, 1].astype(np.float64)
+    xc = np.correlate(b, a, mode="full")
+    k = int(np.argmax(xc))
+    y0, y1, y2 = xc[k - 1], xc[k], xc[k + 1]
+    frac = 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+    lag = (k - (len(a) - 1) + frac) * c.dt
+    expected = 18 * c.spacing / c.vs
+    assert lag == pytest.approx(expected, rel=0.005), (lag, expected)
+
+
 def boundary_ratio(c, src, steps, every=50):
     o = O.Oracle3D(c, 32)
     peak = last = 0.0
