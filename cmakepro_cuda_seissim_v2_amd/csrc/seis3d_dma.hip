@@ -556,9 +556,20 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
             float sxx = cur.o[0] + P.dt * (lam * trace + 2 * mu * e00);
             float syy = cur.o[1] + P.dt * (lam * trace + 2 * mu * e11);
             float szz = surface ? 0.0f : cur.o[2] + P.dt * (lam * trace + 2 * mu * d22);
-            const float sxy = cur.o[3] + P.dt * shear_harmonic(mu, cur.r[1], cur.r[2], cur.r[4]) * p01[0];
-            const float syz = cur.o[4] + P.dt * shear_harmonic(mu, cur.r[2], cur.r[3], cur.r[5]) * (d12 + p21[0]);
-            const float sxz = cur.o[5] + P.dt * shear_harmonic(mu, cur.r[1], cur.r[3], cur.r[6]) * (d02 + p20[0]);
+            // shearMu3D's reciprocals of the seven moduli around the cell, each divided once
+            // (the three harmonic means share them; the same IEEE quotients, summed in the
+            // same order).
+            float inv[7];
+            inv[0] = 1 / mu;
+#pragma unroll
+            for (int j = 1; j < 7; ++j) inv[j] = 1 / cur.r[j];
+            auto shear = [&](int a, int b, int c) {
+                return (mu > 0 && cur.r[a] > 0 && cur.r[b] > 0 && cur.r[c] > 0)
+                           ? 4 / (inv[0] + inv[a] + inv[b] + inv[c]) : 0.0f;
+            };
+            const float sxy = cur.o[3] + P.dt * shear(1, 2, 4) * p01[0];
+            const float syz = cur.o[4] + P.dt * shear(2, 3, 5) * (d12 + p21[0]);
+            const float sxz = cur.o[5] + P.dt * shear(1, 3, 6) * (d02 + p20[0]);
             if (P.src.active) {
                 const int r2 = source_r2d(P.src, x, y, z);
                 if (r2 >= 0) {
