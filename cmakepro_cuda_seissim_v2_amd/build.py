@@ -74,8 +74,8 @@ def build_oracle() -> Path:
     return out
 
 
-CPP_TESTS = {"seis3d_boundary_regression": "seis3d", "seis2d_solver_regression": "seis2d",
-             "seis3d_io_regression": "seis3d", "seis2d_record_regression": "seis2d"}
+CPP_TESTS = {"dropin_session": ["seis3d", "seis2d"], "seis3d_io_regression": ["seis3d"],
+             "seis2d_record_regression": ["seis2d"]}
 
 
 def build_cpp_tests() -> list[Path]:
@@ -84,10 +84,11 @@ def build_cpp_tests() -> list[Path]:
     out_dir = src_dir / "bin"
     out_dir.mkdir(parents=True, exist_ok=True)
     outs = []
-    for name, lib in CPP_TESTS.items():
+    for name, libs in CPP_TESTS.items():
         out = out_dir / name
         _run([CXX, "-O2", "-std=c++17", "-Wall", f"-I{REPO_DIR / 'include'}", str(src_dir / f"{name}.cpp"),
-              "-o", str(out), f"-L{LIB_DIR}", f"-l{lib}", f"-Wl,-rpath,$ORIGIN/../../../cmakepro_cuda_seissim_v2_amd/lib"])
+              "-o", str(out), f"-L{LIB_DIR}", *[f"-l{lib}" for lib in libs],
+              f"-Wl,-rpath,$ORIGIN/../../../cmakepro_cuda_seissim_v2_amd/lib"])
         outs.append(out)
     return outs
 
