@@ -179,7 +179,7 @@ int seis3d_abc_param_count(void) { return seis3d::abc::kParams; }
 
 int seis3d_abc_attach(seis3d_t* t, const float* params) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "attaching the learned boundary");
         const seis3d_ctx& c = t->c;
         if (c.reflecting || c.pml != 8)
             invalid("Seis3D: the learned boundary needs an 8-cell absorbing frame (pml = 8, no reflecting walls).");
@@ -214,7 +214,7 @@ int seis3d_abc_attach(seis3d_t* t, const float* params) {
 
 int seis3d_abc_detach(seis3d_t* t) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "detaching the learned boundary");
         if (!t->abc) return;
         check(hipStreamSynchronize(t->stream), "detach learned boundary");
         release_abc(t->abc);

@@ -165,7 +165,14 @@ Plan plan_for(const seis3d_ctx& c) {
 namespace seis3d::host {
 
 void require_session(const seis3d_t* t) {
-    if (!t || !t->stream) fail(SEIS3D_ELIFECYCLE, "Seis3D: simulation not initialized.");
+    if (!t || !t->stream || t->destroy_pending) fail(SEIS3D_ELIFECYCLE, "Seis3D: simulation not initialized.");
+}
+
+void require_unbatched(const seis3d_t* t, const char* operation) {
+    require_session(t);
+    if (t->batch_refs > 0)
+        fail(SEIS3D_ELIFECYCLE, std::string("Seis3D: ") + operation +
+                                    " is not allowed while a batch holds the session; destroy the batch first.");
 }
 
 void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what) {
@@ -489,6 +496,11 @@ int seis3d_create(const seis3d_ctx* ctx, seis3d_t** out) {
 
 void seis3d_destroy(seis3d_t* t) {
     if (!t) return;
+    if (t->batch_refs > 0) {
+        // A batch graph still holds the session's buffers: free them with the last batch.
+        t->destroy_pending = true;
+        return;
+    }
     t->release();
     delete t;
 }
@@ -516,7 +528,7 @@ int seis3d_reset(seis3d_t* t) {
 // built and uploaded into fresh buffers before the old ones are swapped out.
 int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "a CPML refresh");
         if (!ctx) invalid("Seis3D: null context.");
         seis3d_ctx c = *ctx;
         validate(c);
@@ -561,7 +573,7 @@ int seis3d_read_cpml(seis3d_t* t, int axis, float* out, size_t capacity) {
 // uploadModelPlane3D (cuda/Cuda_Seis3D.cu:696-711).
 int seis3d_upload_plane(seis3d_t* t, int z, const float* values) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "a model upload");
         const seis3d_ctx& c = t->c;
         if (z < 0 || z >= c.nz || !values) invalid("Seis3D: invalid model plane.");
         const std::size_t n = std::size_t(c.nx) * c.ny;
@@ -588,7 +600,7 @@ int seis3d_upload_plane(seis3d_t* t, int z, const float* values) {
 // finishModelUpload3D (cuda/Cuda_Seis3D.cu:714-717).
 int seis3d_finish_upload(seis3d_t* t) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "a model upload");
         check(seis3d::launch_extend_materials(t->base(), t->material_params(), t->stream), "kernel launch");
         check(hipStreamSynchronize(t->stream), "finish model import");
         t->inv_mu_dirty = true;
@@ -673,7 +685,7 @@ int seis3d_synchronize(seis3d_t* t) {
 // setReceivers3D (cuda/Cuda_Seis3D.cu:649-668): validated, all-or-nothing.
 int seis3d_set_receivers(seis3d_t* t, const seis3d_receiver* r, int count) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "replacing receivers");
         const seis3d_ctx& c = t->c;
         if (count < 0 || count > kMaxReceivers || (count && !r)) invalid("Seis3D: invalid receiver count.");
         if (t->recording) fail(SEIS3D_ELIFECYCLE, "Seis3D: stop the gather before replacing receivers.");
@@ -741,7 +753,7 @@ int seis3d_sample_receivers(seis3d_t* t, const float** out) {
 // Recording state of startGather3D (src/Seis3D/Acquisition3D.cpp:188-230).
 int seis3d_gather_begin(seis3d_t* t, int target_samples, int stride) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "starting a gather");
         if (target_samples < 1 || target_samples > kMaxSegySamples || stride < 1 || stride > 10000)
             invalid("Gather sample count must be 1..32767 with stride 1..10000.");
         if (!t->receiver_count) invalid("Seis3D: set receivers before starting a gather.");
@@ -778,7 +790,7 @@ int seis3d_gather_read(seis3d_t* t, float* out, size_t capacity, int* samples) {
 
 int seis3d_gather_end(seis3d_t* t) {
     return guarded([&] {
-        require_session(t);
+        require_unbatched(t, "ending a gather");
         check(hipStreamSynchronize(t->stream), "finish gather");
         (void)hipFree(t->gather);
         t->gather = nullptr;
@@ -1045,8 +1057,12 @@ int seis3d_device_count(void) {
 namespace {
 
 using TileKey = std::tuple<int, int, int, int, int, int>;
+struct Tuned {
+    seis3d::KernelConfig config;
+    double ms_per_step;
+};
 std::mutex g_tune_mutex;
-std::map<TileKey, seis3d::KernelConfig> g_tuned;
+std::map<TileKey, Tuned> g_tuned;
 
 TileKey tile_key(const seis3d_ctx& c) { return {c.gx, c.gy, c.gz, c.pml, c.free_surface, c.reflecting}; }
 
@@ -1079,12 +1095,21 @@ std::vector<seis3d::KernelConfig> tile_candidates(const seis3d_ctx& c) {
 }  // namespace
 
 struct seis3d_batch {
+    // Gather bookkeeping of one session: before the batch's steps and after them.  The
+    // graph writes the same gather slots on every launch, so a launch sets `after`.
+    struct Recording {
+        int count = 0;
+        bool recording = false;
+        bool ghosts = true;
+    };
     std::vector<seis3d_t*> sessions;
+    std::vector<Recording> before, after;
     hipStream_t origin = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> events;
     int64_t steps = 0;
+    bool holds = false;   // sessions' batch_refs were raised by this batch
 
     void release() noexcept {
         if (origin) (void)hipStreamSynchronize(origin);
@@ -1094,6 +1119,16 @@ struct seis3d_batch {
         if (origin) (void)hipStreamDestroy(origin);
         exec = nullptr; graph = nullptr; origin = nullptr;
         events.clear();
+        if (holds) {
+            for (auto* t : sessions) {
+                if (--t->batch_refs == 0 && t->destroy_pending) {
+                    t->release();
+                    delete t;
+                }
+            }
+            holds = false;
+        }
+        sessions.clear();
     }
 };
 
@@ -1111,7 +1146,7 @@ int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* famil
         {
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             auto it = g_tuned.find(key);
-            if (it != g_tuned.end()) { best = it->second; best_t = 0; }
+            if (it != g_tuned.end()) { best = it->second.config; best_t = it->second.ms_per_step * steps; }
         }
         if (best_t < 0) {
             // Time every candidate on a scratch session of the same grid, so the
@@ -1142,7 +1177,7 @@ int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* famil
             }
             seis3d_destroy(scratch);
             std::lock_guard<std::mutex> lock(g_tune_mutex);
-            g_tuned[key] = best;
+            g_tuned[key] = {best, best_t / steps};
         }
         t->kernels = best;
         if (family) *family = best.family;
@@ -1166,22 +1201,39 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
         auto* b = new seis3d_batch();
         b->sessions.assign(sessions, sessions + count);
         b->steps = n_steps;
+        b->before.resize(std::size_t(count));
+        b->after.resize(std::size_t(count));
         try {
             check(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking), "create stream");
             b->events.resize(std::size_t(count) + 1, nullptr);
             for (auto& e : b->events) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+            // Work that depends on session state runs now, outside the graph: 1/mu after a
+            // model change.  Model uploads are refused while the batch exists, so it stays valid.
+            for (int s = 0; s < count; ++s) {
+                seis3d_t* t = b->sessions[s];
+                if (t->inv_mu_dirty) {
+                    check(seis3d::launch_reciprocal_mu(t->base(), t->stream), "kernel launch");
+                    t->inv_mu_dirty = false;
+                }
+                check(hipStreamSynchronize(t->stream), "batch capture");
+                b->before[std::size_t(s)] = {t->gather_count, t->recording, t->ghosts_valid};
+            }
             // Every session's stream joins one capture: fork from the origin stream,
-            // enqueue all steps of every shot on its own stream, join back.
-            for (auto* t : b->sessions) check(hipStreamSynchronize(t->stream), "batch capture");
+            // enqueue all steps of every shot on its own stream, join back.  Each shot
+            // starts by rebuilding its stress ghost planes, so a launch is valid whatever
+            // was written into the wavefield since the last one.
             check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "begin capture");
             check(hipEventRecord(b->events[0], b->origin), "fork capture");
             for (int s = 0; s < count; ++s) {
                 seis3d_t* t = b->sessions[s];
                 check(hipStreamWaitEvent(t->stream, b->events[0], 0), "fork capture");
+                if (t->c.free_surface) check(seis3d::launch_stress_ghost(t->base(), t->stream), "kernel launch");
+                t->ghosts_valid = true;
                 for (int64_t k = 0; k < n_steps; ++k) {
                     enqueue_step(t, sources[s], std::uint64_t(k));
                     maybe_sample(t, std::uint64_t(k));
                 }
+                b->after[std::size_t(s)] = {t->gather_count, t->recording, true};
                 check(hipEventRecord(b->events[s + 1], t->stream), "join capture");
                 check(hipStreamWaitEvent(b->origin, b->events[s + 1], 0), "join capture");
             }
@@ -1193,10 +1245,24 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
                 (void)hipStreamEndCapture(b->origin, &partial);
                 if (partial) (void)hipGraphDestroy(partial);
             }
+            // Capture only recorded work: the gather bookkeeping is as it was.
+            for (std::size_t s = 0; s < b->sessions.size(); ++s) {
+                b->sessions[s]->gather_count = b->before[s].count;
+                b->sessions[s]->recording = b->before[s].recording;
+                b->sessions[s]->ghosts_valid = b->before[s].ghosts;
+            }
             b->release();
             delete b;
             throw;
         }
+        // Nothing has run yet: samples are counted when the batch is launched.
+        for (std::size_t s = 0; s < b->sessions.size(); ++s) {
+            b->sessions[s]->gather_count = b->before[s].count;
+            b->sessions[s]->recording = b->before[s].recording;
+            b->sessions[s]->ghosts_valid = b->before[s].ghosts;
+            ++b->sessions[s]->batch_refs;
+        }
+        b->holds = true;
         *out = b;
     });
 }
@@ -1204,7 +1270,14 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
 int seis3d_batch_launch(seis3d_batch_t* b) {
     return guarded([&] {
         if (!b || !b->exec) fail(SEIS3D_ELIFECYCLE, "Seis3D: batch not created.");
+        for (auto* t : b->sessions)
+            if (t->destroy_pending) fail(SEIS3D_ELIFECYCLE, "Seis3D: a session of this batch was destroyed.");
         check(hipGraphLaunch(b->exec, b->origin), "launch batch");
+        for (std::size_t s = 0; s < b->sessions.size(); ++s) {
+            b->sessions[s]->gather_count = b->after[s].count;
+            b->sessions[s]->recording = b->after[s].recording;
+            b->sessions[s]->ghosts_valid = true;
+        }
     });
 }
 

@@ -416,6 +416,9 @@ int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_
         if (!shots || count < 1 || count > 1024 || !g_vp || !g_vs || !g_rho)
             invalid("Seis3D: invalid FWI gradient request.");
         if (t->recording) fail(SEIS3D_ELIFECYCLE, "Seis3D: stop the gather before computing a gradient.");
+        // The gradient is the adjoint of the plain CPML step; a learned frame would make it
+        // the gradient of a different forward operator than seis3d_run's.
+        if (t->abc) fail(SEIS3D_ELIFECYCLE, "Seis3D: detach the learned boundary before computing a gradient.");
         int most = 0;
         for (int s = 0; s < count; ++s) {
             check_source(c, shots[s].source);

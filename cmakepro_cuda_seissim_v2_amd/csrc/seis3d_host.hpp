@@ -121,6 +121,11 @@ struct seis3d_session {
     bool ghosts_valid = true;          // σ ghost planes consistent with the interior
     seis3d::KernelConfig kernels;
     seis3d::host::AbcFrame* abc = nullptr;   // learned absorbing boundary, when attached
+    // Batches whose captured graph holds this session's device pointers.  While any
+    // exists, calls that free or replace those buffers are refused, and a destroy is
+    // deferred until the last such batch is destroyed.
+    int batch_refs = 0;
+    bool destroy_pending = false;
 
     seis3d::DevStep base() const {
         seis3d::DevStep P{};
@@ -171,6 +176,8 @@ struct seis3d_session {
 namespace seis3d::host {
 
 void require_session(const seis3d_t* t);
+// Refuses (SEIS3D_ELIFECYCLE) an operation that would free or replace device buffers a batch holds.
+void require_unbatched(const seis3d_t* t, const char* operation);
 void alloc_floats(float*& p, std::size_t count, hipStream_t s, const char* what);
 // A zeroed grid array of `cells` floats at kGridLead into its allocation (free with free_grid).
 void alloc_grid(float*& p, std::size_t cells, hipStream_t s, const char* what);

@@ -539,6 +539,10 @@ int seis3d_s3dm_load(seis3d_t* t, const char* path) {
         if (seis3d_s3dm_inspect(path, &c) != SEIS3D_OK) fail(SEIS3D_EINVAL, g_error);
         if (c.nx != t->c.nx || c.ny != t->c.ny || c.nz != t->c.nz)
             invalid("Seis3D: the model's dimensions differ from the session's.");
+        // The session's dt and CPML tables were built for its spacing and boundaries.
+        if (std::memcmp(&c.spacing, &t->c.spacing, sizeof(float)) != 0 || c.pml != t->c.pml ||
+            c.free_surface != t->c.free_surface || c.reflecting != t->c.reflecting)
+            invalid("Seis3D: the model's spacing, CPML thickness or boundaries differ from the session's.");
         std::ifstream in(path, std::ios::binary);
         in.seekg(36);
         std::vector<float> plane(std::size_t(c.nx) * c.ny * 3);

@@ -266,7 +266,15 @@ typedef struct seis3d_batch seis3d_batch_t;
  * an open gather) into one HIP graph: each shot runs on its own stream, the
  * graph is instantiated once and replayed with one launch.  Replaces the
  * sequential shot loop of the reference (src/Seis3D_Sim.cpp:133-143,
- * src/Seis3D/Acquisition3D.cpp:233-242). */
+ * src/Seis3D/Acquisition3D.cpp:233-242).
+ * The graph holds each session's device buffers.  While a batch exists, calls
+ * that would free or replace them (seis3d_refresh_cpml, seis3d_upload_plane,
+ * seis3d_finish_upload, seis3d_set_receivers, seis3d_gather_begin/_end,
+ * seis3d_abc_attach/_detach) return SEIS3D_ELIFECYCLE, and seis3d_destroy of a
+ * batched session is deferred until its last batch is destroyed.  Samples are
+ * counted into each session's gather when the batch is launched; every launch
+ * runs steps 0..n_steps-1 from the sessions' current wavefields and writes the
+ * same gather slots. */
 int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources, int count,
                         int64_t n_steps, seis3d_batch_t** out);
 int seis3d_batch_launch(seis3d_batch_t* b);

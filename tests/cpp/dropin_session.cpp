@@ -15,8 +15,9 @@
 //      symmetric under x -> -x, y -> -y and z -> -z, so collocated vx at +d equals -vx at -d
 //      (and likewise for y and z), and the three axes see the same trace (the stencil and
 //      the CPML do not prefer an axis);
-//    - a 2D Type-1 run is deterministic and its explosive source radiates vz symmetrically
-//      about the source column.
+//    - a 2D Type-1 run is deterministic, and its explosive source radiates vz oddly about the
+//      source row (single-source mode: a negative active-source count, as
+//      runGpuSimulationStep takes it).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -150,7 +151,7 @@ std::vector<float> run2D(int size, int npml, const std::vector<int>& receivers, 
     initializeGpuSimulation(g, c);
     std::vector<float> vx(receivers.size() * std::size_t(nt)), vz(vx.size());
     for (int it = 0; it < nt; ++it) {
-        runGpuSimulationStep(g, it, c, 0);
+        runGpuSimulationStep(g, it, c, -1);   // a negative count selects the single source
         copyReceiverRecordsFromGpu(g, c.num_rcv, nt, it, vx.data(), vz.data());
     }
     releaseGpuSimulation(g);
@@ -158,8 +159,14 @@ std::vector<float> run2D(int size, int npml, const std::vector<int>& receivers, 
 }
 
 void explosion2D() {
-    const int size = 161, npml = 20, nt = 500, mid = size / 2, d = 30;
-    const std::vector<int> rec = {mid * size + mid + d, mid * size + mid - d};
+    // The split PML damps vz with the half-cell profile, which is not mirror symmetric, so the
+    // square is large enough that nothing returns from the PML within the 0.25 s record.
+    const int size = 241, npml = 20, nt = 500, mid = size / 2, d = 30;
+    // vz sits half a cell below its node, (x, z + 1/2), and an explosion's vz is odd about the
+    // source row: the image of row mid + d is row mid - d - 1, with the opposite sign.  (Along
+    // the source row itself vz is a small difference of the two large split halves, where FP32
+    // rounding dominates, so the vertical pair is the sharper check.)
+    const std::vector<int> rec = {(mid + d) * size + mid, (mid - d - 1) * size + mid};
     const auto a = run2D(size, npml, rec, nt);
     const auto b = run2D(size, npml, rec, nt);
     expect(a == b, "two identical 2D runs differ");
@@ -168,11 +175,11 @@ void explosion2D() {
         const double p = a[std::size_t(it)], q = a[std::size_t(nt) + it];
         expect(std::isfinite(p) && std::isfinite(q), "non-finite 2D record");
         peak = std::max(peak, std::abs(p));
-        asym = std::max(asym, std::abs(p - q));
+        asym = std::max(asym, std::abs(p + q));
     }
-    std::printf("2D vz peak %.6e left/right spread %.3e\n", peak, asym / peak);
+    std::printf("2D vz peak %.6e up/down antisymmetry %.3e\n", peak, asym / peak);
     expect(peak > 0, "no 2D signal at the receivers");
-    expect(asym <= 1e-5 * peak, "vz is not even about the source column");
+    expect(asym <= 2e-5 * peak, "vz is not odd about the source row");
 }
 
 }  // namespace
