@@ -26,8 +26,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s 3D O(8) VS-FD + %HBM roofline; batched FWI-grad shots/s vs plain"
-FAMILY_NAMES = {0: "direct", 1: "stream-2.5D", 2: "pipelined-2.5D", 3: "column", 4: "lds-dma-2.5D"}
-FAMILY_KERNEL = {0: "direct", 1: "stream", 2: "pipe", 3: "column", 4: "dma"}
+FAMILY_NAMES = {0: "direct", 4: "lds-dma-2.5D"}
+FAMILY_KERNEL = {0: "direct", 4: "dma"}
 
 
 def parse():
@@ -49,60 +49,39 @@ def parse():
     p.add_argument("--fd", type=int, default=0,
                    help="FWI: central-difference checks along the gradient (vp, vs, rho, all three)")
     p.add_argument("--no-copy-ceiling", action="store_true",
-                   help="skip the device-copy ceiling measurement (profiled runs)")
+                   help="skip the stream-kernel ceiling measurement (profiled runs)")
     p.add_argument("--copy-ceiling", action="store_true",
-                   help="instead of the bench, print the measured device-copy bandwidth (GB/s, read + write)")
+                   help="instead of the bench, print the measured stream-kernel ceilings (GB/s)")
+    p.add_argument("--no-extra", action="store_true",
+                   help="default line only: skip its grid_200 and fwi objects (profiled runs)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                   help="instead of the bench, run 5 plain device copies of MIB MiB (counter calibration)")
+                   help="instead of the bench, run 5 launches of the library's copy kernel of MIB MiB (counter calibration)")
     return p.parse_args()
 
 
 def copy_probe(mib: int) -> None:
-    """Plain device-to-device copies of a known size, used to calibrate rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters against a kernel whose HBM traffic is known."""
-    import torch
-    n = mib * 1024 * 1024 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+    """Five launches of this library's copy kernel over arrays of a known size, to calibrate
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters against a kernel whose HBM traffic is known."""
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    S.bandwidth_probe("copy", mib, 5)
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": mib << 20, "bytes_written_per_copy": mib << 20}))
 
 
-def copy_ceiling_gbs(mib: int = 1024, reps: int = 10) -> float:
-    """Measured HBM ceiling: bytes read + written per second by a plain 1 GiB device copy.
-
-    Runs torch's device copy, so it must run in a process where the solver library has not
-    initialised the GPU (torch brings its own HIP runtime); see measured_copy_ceiling()."""
-    import torch
-    n = mib * 1024 * 1024 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    e1.synchronize()
-    secs = e0.elapsed_time(e1) / 1e3 / reps
-    del a, b
-    torch.cuda.empty_cache()
-    return 2 * n * 4 / secs / 1e9
-
-
-def measured_copy_ceiling():
-    """copy_ceiling_gbs() in a child process (GB/s), or None if it fails."""
-    import subprocess
-    try:
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-ceiling"], capture_output=True,
-                           text=True, timeout=300)
-        return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_ceiling_gbs"])
-    except Exception as e:  # the bench line still carries the paper peak
-        print(f"bench: copy ceiling unavailable ({e})", file=sys.stderr, flush=True)
-        return None
+def measured_ceilings() -> dict:
+    """The measured HBM ceilings: this library's own stream kernels (seis3d_bandwidth_probe), GB/s
+    of bytes moved (read + written).  read/write/copy use 16-byte lanes; copy_lds_dma stages
+    through LDS with global_load_lds like the step kernels; mix10_3 and mix12_6 stream the
+    velocity and stress half-steps' numbers of arrays elementwise."""
+    from cmakepro_cuda_seissim_v2_amd import seis3d as S
+    out = {}
+    for kind, mib in (("read", 1024), ("write", 1024), ("copy", 1024), ("copy_lds_dma", 1024), ("mix10_3", 256),
+                      ("mix12_6", 256)):
+        try:
+            out[kind] = round(S.bandwidth_probe(kind, mib, 10)["gbs"], 1)
+        except Exception as e:  # the line still carries the paper peak
+            print(f"bench: probe {kind} unavailable ({e})", file=sys.stderr, flush=True)
+    print(f"bench: measured ceilings GB/s {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def profiled_traffic(kernel: str, grid: str):
@@ -493,45 +472,59 @@ def cpu_info():
 
 
 def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
-    """FP32 CPU oracle (OpenMP over z) on the same workload, bounded to ~10-30 s: all host
-    cores (the value), one thread, and the host STREAM triad as the CPU's bandwidth ceiling."""
+    """FP32 CPU oracle (OpenMP over z) on the same workload, bounded to ~15-30 s: the median of 5
+    timed samples on the box's CPU share (at most 16 threads, the share one GPU's job gets) is the
+    value; the median of 3 one-thread samples and the host STREAM triad are reported beside it."""
+    import statistics
     from oracle import pyoracle as O
     cores, cpu_model = cpu_info()
+    nproc = os.cpu_count() or cores
     threads = min(cores, 16)
     O.set_threads(threads)
     o = O.Oracle3D(ctx, 32)
     o.upload_model(*model)
     o.set_receivers(rec)
     warm = o.time_steps(src, 0, 1)
-    steps = steps_hint if steps_hint > 0 else max(2, min(60, int(15.0 / max(warm, 1e-3))))
-    secs = o.time_steps(src, 1, steps)
-    snapshot = [o.field(f) for f in range(9)]   # the state after steps 0..steps, for the parity check
+    per = steps_hint if steps_hint > 0 else max(1, min(40, int(3.0 / max(warm, 1e-3))))
+    samples, done = [], 1
+    for _ in range(5):
+        samples.append(o.time_steps(src, done, per) / per)
+        done += per
+    snapshot = [o.field(f) for f in range(9)]   # the state after steps 0..done-1, for the parity check
     O.set_threads(1)
-    one_steps = 2
-    one_secs = o.time_steps(src, 1 + steps, one_steps)
+    one = []
+    for _ in range(3):
+        one.append(o.time_steps(src, done, 1))
+        done += 1
     o.close()
     O.set_threads(threads)
     triad = O.stream_triad(1 << 26, threads)   # 3 x 512 MiB of doubles: far beyond the caches
     n_upd = (ctx.gx - 8) * (ctx.gy - 8) * (ctx.gz - 8)
     step_bytes = 4.0 * (30 * n_upd + 12 * n_psi)
-    cpu_gbs = step_bytes * steps / secs / 1e9
-    print(f"bench: cpu oracle {threads} threads {steps} steps {secs:.2f} s, 1 thread {one_steps} steps "
-          f"{one_secs:.2f} s, host triad {triad:.1f} GB/s", file=sys.stderr, flush=True)
+    t_step = statistics.median(samples)
+    t_one = statistics.median(one)
+    cpu_gbs = step_bytes / t_step / 1e9
+    print(f"bench: cpu oracle {threads} threads: {per} steps x 5, median {t_step:.3f} s/step; 1 thread median "
+          f"{t_one:.3f} s/step; host triad {triad:.1f} GB/s; nproc {nproc}", file=sys.stderr, flush=True)
     baseline = {
-        "value": round(n_upd * steps / secs / 1e9, 5),
+        "value": round(n_upd / t_step / 1e9, 5),
         "unit": "Gcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "one_thread_value": round(n_upd * one_steps / one_secs / 1e9, 5),
+        "host_nproc": nproc,
+        "host_cpu_share": cores,
+        "samples_s_per_step": [round(v, 4) for v in samples],
+        "one_thread_value": round(n_upd / t_one / 1e9, 5),
         "stream_triad_gbs": round(triad, 1),
         "algorithmic_gbs": round(cpu_gbs, 2),
         "frac_of_stream_triad": round(cpu_gbs / triad, 3) if triad > 0 else None,
         "sample": f"FP32 oracle (line-faithful C++ restatement, OpenMP {threads} threads, -O3 -march=x86-64-v3, "
-                  f"no FP contraction) on the same 300x50x300 workload, steps 1..{steps} after 1 warm-up step, "
-                  f"{secs:.2f} s; then 1 thread for {one_steps} steps, {one_secs:.2f} s; host STREAM triad "
-                  f"(3 x 512 MiB doubles, {threads} threads); host CPU: {cpu_model}",
+                  f"no FP contraction) on the same {ctx.nx}x{ctx.ny}x{ctx.nz} workload: 1 warm-up step, then 5 "
+                  f"samples of {per} steps, median per step; 1 thread: median of 3 single steps; host STREAM triad "
+                  f"(3 x 512 MiB doubles, {threads} threads); host: {nproc} logical CPUs, {cores} in this "
+                  f"process's affinity set, {threads} used (the CPU share of one GPU's job); CPU: {cpu_model}",
     }
-    return baseline, snapshot, steps + 1
+    return baseline, snapshot, done - 3
 
 
 def snapshot_parity(ctx, model, src, oracle32, steps):
@@ -579,7 +572,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.copy_ceiling:
-        print(json.dumps({"copy_ceiling_gbs": copy_ceiling_gbs()}), flush=True)
+        print(json.dumps(measured_ceilings()), flush=True)
         return
     if args.workload == "batch":
         if args.steps == 1000:
@@ -598,53 +591,23 @@ def main():
         fwi_bench(args)
         return
 
-    # Before this process touches the GPU: the copy ceiling runs in a child with torch's runtime.
-    ceiling = None if args.no_copy_ceiling else measured_copy_ceiling()
-
     from cmakepro_cuda_seissim_v2_amd import seis3d as S
     from cmakepro_cuda_seissim_v2_amd import workloads as W
 
+    ceilings = {} if args.no_copy_ceiling else measured_ceilings()
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     ctx = W.config2_context(nx, ny, nz)
     model = W.layered_model(ctx.nx, ctx.ny, ctx.nz)
     src = W.config2_source(ctx)
     rec = W.config2_receivers(ctx)
-    sb = S.step_bytes(ctx)
-
-    g = S.GPUSimData3D(ctx)
-    if args.tile_y or args.z_chunk or args.family >= 0:
-        f0, ty0, zc0 = g.get_tiles()
-        g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk or zc0)
-    family, tile_y, z_chunk = g.get_tiles()
-    g.upload_model(*model)
-    g.set_receivers(rec)
-    stride = 2
-    total_steps = args.warmup + args.steps
-    g.gather_begin(max(1, total_steps // stride), stride)
-
-    print(f"bench: warmup {args.warmup} steps", file=sys.stderr, flush=True)
-    g.run(src, 0, args.warmup)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.run(src, args.warmup, args.steps)
-    g.synchronize()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    gather = g.gather_read()
-    print(f"bench: timed {args.steps} steps in {secs:.4f} s; gather {gather.shape}", file=sys.stderr, flush=True)
-
-    # Per-kernel durations with HIP events on the session stream.
-    g.gather_end()
-    kt = g.run_timed(src, total_steps, args.kernel_steps)
-    g.synchronize()
-    vel_us = kt.velocity_ms * 1e3 / kt.launches
-    str_us = kt.stress_ms * 1e3 / kt.launches
+    run = step_run(args, S, ctx, model, src, rec, args.steps, args.warmup)
+    sb, vel_us, str_us = run["bytes"], run["velocity_us"], run["stress_us"]
     vel_gbs = sb["velocity"] / (vel_us * 1e-6) / 1e9
     str_gbs = sb["stress"] / (str_us * 1e-6) / 1e9
     dominant = ("stress", str_us, sb["stress"], str_gbs) if str_us >= vel_us else ("velocity", vel_us, sb["velocity"], vel_gbs)
-    g.release()
-
+    family, tile_y, z_chunk = run["tiles"]
     n_upd = sb["n_upd"]
+    secs = run["secs"]
     value = n_upd * args.steps / secs / 1e9
     step_gbs = sb["total"] * args.steps / secs / 1e9
     out = {
@@ -671,6 +634,8 @@ def main():
             "step_frac_of_peak": round(step_gbs / HBM_PEAK_GBS, 4),
             "velocity_kernel_us": round(vel_us, 2), "stress_kernel_us": round(str_us, 2),
             "velocity_kernel_gbs": round(vel_gbs, 1), "stress_kernel_gbs": round(str_gbs, 1),
+            "velocity_frac_of_peak": round(vel_gbs / HBM_PEAK_GBS, 4),
+            "stress_frac_of_peak": round(str_gbs / HBM_PEAK_GBS, 4),
         },
         "roofline": {
             "bound": "hbm",
@@ -686,9 +651,10 @@ def main():
         },
         "cpu_baseline": None,
     }
-    if ceiling:
-        out["roofline"]["copy_ceiling_gbs"] = round(ceiling, 1)
-        out["roofline"]["frac_of_copy_ceiling"] = round(dominant[3] / ceiling, 4)
+    if ceilings:
+        out["roofline"]["measured_ceilings_gbs"] = ceilings
+        if ceilings.get("copy"):
+            out["roofline"]["frac_of_copy_ceiling"] = round(dominant[3] / ceilings["copy"], 4)
     tb, src_file = profiled_traffic(out["roofline"]["kernel"], args.grid)
     if tb:
         out["roofline"]["traffic"] = tb
@@ -698,7 +664,127 @@ def main():
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"], snapshot, snap_steps = cpu_baseline(ctx, model, src, rec, args.cpu_steps, sb["n_psi"])
         out["parity"] = snapshot_parity(ctx, model, src, snapshot, snap_steps)
+    if not args.no_extra:
+        out["grid_200"] = grid_200(args, S, W)
+        out["fwi"] = fwi_leg(S, W)
     print(json.dumps(out), flush=True)
+
+
+def step_run(args, S, ctx, model, src, rec, steps, warmup):
+    """Time `steps` steps (after `warmup`) with the default kernels, gathers recorded on the device;
+    then per-kernel durations with HIP events on the session stream."""
+    sb = S.step_bytes(ctx)
+    g = S.GPUSimData3D(ctx)
+    if args.tile_y or args.z_chunk or args.family >= 0:
+        f0, ty0, zc0 = g.get_tiles()
+        g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk or zc0)
+    tiles = g.get_tiles()
+    g.upload_model(*model)
+    g.set_receivers(rec)
+    stride = 2
+    total_steps = warmup + steps
+    g.gather_begin(max(1, total_steps // stride), stride)
+    print(f"bench: {ctx.nx}x{ctx.ny}x{ctx.nz} warmup {warmup} steps", file=sys.stderr, flush=True)
+    g.run(src, 0, warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(src, warmup, steps)
+    g.synchronize()
+    secs = time.perf_counter() - t0
+    gather = g.gather_read()
+    print(f"bench: timed {steps} steps in {secs:.4f} s; gather {gather.shape}", file=sys.stderr, flush=True)
+    g.gather_end()
+    kt = g.run_timed(src, total_steps, args.kernel_steps)
+    g.synchronize()
+    g.release()
+    return {"bytes": sb, "secs": secs, "tiles": tiles, "velocity_us": kt.velocity_ms * 1e3 / kt.launches,
+            "stress_us": kt.stress_ms * 1e3 / kt.launches}
+
+
+def grid_200(args, S, W) -> dict:
+    """The 200^3 grid with config 2's boundaries, source and receivers: throughput, per-kernel
+    fraction of the HBM peak, and its error against both oracles after a bounded 41 steps."""
+    import numpy as np
+    from oracle import pyoracle as O
+    ctx = W.config2_context(200, 200, 200)
+    model = W.layered_model(200, 200, 200)
+    src = W.config2_source(ctx)
+    rec = W.config2_receivers(ctx)
+    steps = max(args.steps, 100)
+    run = step_run(args, S, ctx, model, src, rec, steps, 10)
+    sb = run["bytes"]
+    snap = 41
+    g = S.GPUSimData3D(ctx)
+    g.upload_model(*model)
+    g.run(src, 0, snap)
+    core = (slice(4, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
+    gpu = [g.download_field(f)[core].astype(np.float64) for f in range(9)]
+    g.release()
+    errs = {}
+    O.set_threads(min(16, cpu_info()[0]))
+    for precision in (32, 64):
+        with O.Oracle3D(ctx, precision) as o:
+            o.upload_model(*model)
+            o.time_steps(src, 0, snap)
+            errs[precision] = max(float(np.linalg.norm(a - o.field(f)[core]) / max(np.linalg.norm(o.field(f)[core]), 1e-300))
+                                  for f, a in enumerate(gpu))
+    out = {
+        "workload": "200x200x200, dx=10 m, free top + 20-cell CPML, 15 Hz force, 150 receivers stride 2",
+        "n_upd": sb["n_upd"], "n_psi": sb["n_psi"], "steps": steps,
+        "gcell_updates_per_s": round(sb["n_upd"] * steps / run["secs"] / 1e9, 4),
+        "ms_per_step": round(run["secs"] / steps * 1e3, 5),
+        "velocity_kernel_us": round(run["velocity_us"], 2), "stress_kernel_us": round(run["stress_us"], 2),
+        "velocity_frac_of_peak": round(sb["velocity"] / (run["velocity_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+        "stress_frac_of_peak": round(sb["stress"] / (run["stress_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+        "parity": {"steps": snap, "snapshot_rel_l2_vs_fp32_oracle": float(f"{errs[32]:.3e}"),
+                   "snapshot_rel_l2_vs_fp64_oracle": float(f"{errs[64]:.3e}"),
+                   "tolerance_fp32": 1e-5, "tolerance_fp64": 1e-4},
+    }
+    print(f"bench: grid_200 {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def fwi_leg(S, W) -> dict:
+    """The metric's FWI half at full size: 8 shots on 200x100x200 (free top), 1200 steps,
+    checkpoints every 50 steps; batched request vs the plain per-shot loop, both through
+    seis3d_fwi_gradient, and the misfit."""
+    import numpy as np
+    steps, stride, interval = 1200, 2, 50
+    ctx, start, true, shots = W.fwi_workload(8)
+    observed = []
+    with S.GPUSimData3D(ctx) as g:
+        g.upload_model(*true)
+        for src, rec in shots:
+            g.reset()
+            g.set_receivers(rec)
+            g.gather_begin(steps // stride, stride)
+            g.run(src, 0, steps)
+            observed.append(g.gather_read())
+            g.gather_end()
+    req = [(src, rec, obs) for (src, rec), obs in zip(shots, observed)]
+    with S.GPUSimData3D(ctx) as g:
+        g.upload_model(*start)
+        g.synchronize()
+        t0 = time.perf_counter()
+        batched = g.fwi_gradient(req, steps, stride, interval)
+        t_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        singles = [g.fwi_gradient([r], steps, stride, interval) for r in req]
+        t_plain = time.perf_counter() - t0
+    summed = [np.sum([s[k].astype(np.float64) for s in singles], axis=0) for k in (1, 2, 3)]
+    agree = max(float(np.linalg.norm(batched[k + 1] - summed[k]) / np.linalg.norm(summed[k])) for k in range(3))
+    out = {
+        "workload": "FWI gradient (vp, vs, rho), 8 shots, 200x100x200, dx=10 m, free top + 20-cell CPML, 1200 steps, "
+                    "15 Hz explosions, 200 surface receivers stride 2, checkpoint every 50 steps",
+        "batched_shots_per_s": round(len(shots) / t_batch, 4),
+        "plain_shots_per_s": round(len(shots) / t_plain, 4),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
+        "misfit": batched[0],
+        "batched_vs_summed_singles_rel_l2": float(f"{agree:.3e}"),
+    }
+    print(f"bench: fwi {out}", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -276,6 +276,56 @@ void read_flag(seis3d_t* t, unsigned int* flag) {
           "check capture stability");
 }
 
+long long view_samples(const seis3d_ctx& c, int volume) {
+    const auto l = make_view_layout(c);
+    return static_cast<long long>(volume ? l.total : l.volume_offset);
+}
+
+// extractViews3D (cuda/Cuda_Seis3D.cu:627-646) through device staging the caller provides:
+// `stage` holds 2 floats per sample, `stat_words` 5 words.  Synchronises the session stream.
+void extract_views_staged(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
+                          int material_component, int volume, int include_boundary, seis3d_view_stats* stats,
+                          float* stage, unsigned* stat_words) {
+    const seis3d_ctx& c = t->c;
+    if (!out || !slices || material_component < 5 || material_component > 7 || component < 0 || component > 7 ||
+        slices[0] < 0 || slices[0] >= c.nx || slices[1] < 0 || slices[1] >= c.ny || slices[2] < 0 ||
+        slices[2] >= c.nz)
+        invalid("Seis3D: invalid view parameters.");
+    const auto l = make_view_layout(c);
+    const long long count = view_samples(c, volume);
+    if (capacity < std::size_t(count) * 2) invalid("Seis3D: view buffer too small.");
+    seis3d::ViewParams vp{};
+    for (int a = 0; a < 3; ++a) {
+        vp.slice_offset[a] = static_cast<long long>(l.slice_offset[a]);
+        vp.volume[a] = l.volume[a];
+        vp.domain[a] = l.domain[a];
+        vp.origin[a] = l.origin[a];
+        vp.slices[a] = slices[a];
+    }
+    vp.volume_offset = static_cast<long long>(l.volume_offset);
+    vp.total = static_cast<long long>(l.total);
+    vp.nx = c.nx; vp.ny = c.ny; vp.nz = c.nz;
+    vp.ox = c.ox; vp.oy = c.oy; vp.oz = c.oz;
+    vp.component = component;
+    vp.material_component = material_component;
+    vp.include_boundary = include_boundary != 0;
+    static const unsigned init[5] = {0u, 0u, 0x7f7fffffu, 0u, 0u};
+    check(hipMemcpyAsync(stat_words, init, sizeof(init), hipMemcpyHostToDevice, t->stream), "reset view statistics");
+    check(seis3d::launch_extract_views(t->base(), vp, count, stage, stat_words, t->stream), "kernel launch");
+    check(hipMemcpyAsync(out, stage, std::size_t(count) * 2 * sizeof(float), hipMemcpyDeviceToHost, t->stream),
+          "read view");
+    unsigned host[5]{};
+    check(hipMemcpyAsync(host, stat_words, sizeof(host), hipMemcpyDeviceToHost, t->stream), "read diagnostics");
+    check(hipStreamSynchronize(t->stream), "finish view extraction");
+    if (host[1]) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; session stopped.");
+    if (stats) {
+        std::memcpy(&stats->maximum, host + 0, 4);
+        std::memcpy(&stats->material_min, host + 2, 4);
+        std::memcpy(&stats->material_max, host + 3, 4);
+        std::memcpy(&stats->velocity_maximum, host + 4, 4);
+    }
+}
+
 }  // namespace seis3d::host
 
 using namespace seis3d::host;
@@ -888,72 +938,31 @@ int seis3d_extract_views(seis3d_t* t, float* out, size_t capacity, const int32_t
                          int material_component, int volume, int include_boundary, seis3d_view_stats* stats) {
     return guarded([&] {
         require_session(t);
-        const seis3d_ctx& c = t->c;
-        if (!out || !slices || material_component < 5 || material_component > 7 || component < 0 || component > 7 ||
-            slices[0] < 0 || slices[0] >= c.nx || slices[1] < 0 || slices[1] >= c.ny || slices[2] < 0 ||
-            slices[2] >= c.nz)
-            invalid("Seis3D: invalid view parameters.");
-        const auto l = make_view_layout(c);
-        const long long count = static_cast<long long>(volume ? l.total : l.volume_offset);
-        if (capacity < std::size_t(count) * 2) invalid("Seis3D: view buffer too small.");
-        seis3d::ViewParams vp{};
-        for (int a = 0; a < 3; ++a) {
-            vp.slice_offset[a] = static_cast<long long>(l.slice_offset[a]);
-            vp.volume[a] = l.volume[a];
-            vp.domain[a] = l.domain[a];
-            vp.origin[a] = l.origin[a];
-            vp.slices[a] = slices[a];
-        }
-        vp.volume_offset = static_cast<long long>(l.volume_offset);
-        vp.total = static_cast<long long>(l.total);
-        vp.nx = c.nx; vp.ny = c.ny; vp.nz = c.nz;
-        vp.ox = c.ox; vp.oy = c.oy; vp.oz = c.oz;
-        vp.component = component;
-        vp.material_component = material_component;
-        vp.include_boundary = include_boundary != 0;
+        const long long count = view_samples(t->c, volume);
         float* d = nullptr;
         unsigned* st = nullptr;
         check(hipMalloc(&d, std::size_t(count) * 2 * sizeof(float)), "allocate view");
         try {
             check(hipMalloc(&st, 5 * sizeof(unsigned)), "allocate view statistics");
-            const unsigned init[5] = {0u, 0u, 0x7f7fffffu, 0u, 0u};
-            check(hipMemcpyAsync(st, init, sizeof(init), hipMemcpyHostToDevice, t->stream), "reset view statistics");
-            check(seis3d::launch_extract_views(t->base(), vp, count, d, st, t->stream), "kernel launch");
-            check(hipMemcpyAsync(out, d, std::size_t(count) * 2 * sizeof(float), hipMemcpyDeviceToHost, t->stream),
-                  "read view");
-            unsigned host[5]{};
-            check(hipMemcpyAsync(host, st, sizeof(host), hipMemcpyDeviceToHost, t->stream), "read diagnostics");
-            check(hipStreamSynchronize(t->stream), "finish view extraction");
-            (void)hipFree(d);
-            (void)hipFree(st);
-            d = nullptr;
-            st = nullptr;
-            if (host[1]) fail(SEIS3D_ENONFINITE, "Seis3D: non-finite wavefield detected; session stopped.");
-            if (stats) {
-                std::memcpy(&stats->maximum, host + 0, 4);
-                std::memcpy(&stats->material_min, host + 2, 4);
-                std::memcpy(&stats->material_max, host + 3, 4);
-                std::memcpy(&stats->velocity_maximum, host + 4, 4);
-            }
+            extract_views_staged(t, out, capacity, slices, component, material_component, volume, include_boundary,
+                                 stats, d, st);
         } catch (...) {
             (void)hipFree(d);
             (void)hipFree(st);
             throw;
         }
+        (void)hipFree(d);
+        (void)hipFree(st);
     });
 }
 
 int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk) {
     return guarded([&] {
         require_session(t);
-        if (family < 0 || family > 4)
-            invalid("Seis3D: kernel family must be 0 (direct), 1 (stream), 2 (pipelined), 3 (column) or 4 (LDS-DMA).");
-        if (family == 1 && tile_y != 4 && tile_y != 8 && tile_y != 16) invalid("Seis3D: tile_y must be 4, 8 or 16.");
-        if ((family == 2 || family == 4) && tile_y != 4 && tile_y != 8) invalid("Seis3D: tile_y must be 4 or 8.");
-        if (family == 4 && z_chunk > 128) invalid("Seis3D: z_chunk must be at most 128 for the LDS-DMA family.");
-        if (family == 3 && tile_y != 1 && tile_y != 2 && tile_y != 4 && tile_y != 8)
-            invalid("Seis3D: tile_y must be 1, 2, 4 or 8.");
-        if (family >= 1 && (z_chunk < 1 || z_chunk > 4096)) invalid("Seis3D: z_chunk must be 1..4096.");
+        if (family != 0 && family != 4) invalid("Seis3D: kernel family must be 0 (direct) or 4 (LDS-DMA).");
+        if (family == 4 && tile_y != 4 && tile_y != 8) invalid("Seis3D: tile_y must be 4 or 8.");
+        if (family == 4 && (z_chunk < 1 || z_chunk > 128))
+            invalid("Seis3D: z_chunk must be 1..128 for the LDS-DMA family.");
         t->kernels.family = family;
         if (family >= 1) {
             t->kernels.tile_y = tile_y;

@@ -189,5 +189,12 @@ seis3d::DevSource make_source(const seis3d_ctx& c, const seis3d_source& s, doubl
 void enqueue_step(seis3d_t* t, const seis3d_source& s, std::uint64_t n, hipEvent_t* ev = nullptr);
 // Learned-boundary correction of the frame cells after the stress half-step (no-op when detached).
 void enqueue_abc(seis3d_t* t);
+// Samples of the packed views (planes, plus the volume when `volume`), 2 floats each.
+long long view_samples(const seis3d_ctx& c, int volume);
+// extractViews3D through caller-provided device staging (2 floats per sample) and 5 statistics
+// words; validates the request and synchronises the session stream.
+void extract_views_staged(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
+                          int material_component, int volume, int include_boundary, seis3d_view_stats* stats,
+                          float* stage, unsigned* stat_words);
 
 }  // namespace seis3d::host

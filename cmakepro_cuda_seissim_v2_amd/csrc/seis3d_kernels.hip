@@ -3,17 +3,10 @@
 // injection fused in), free-surface ghost planes, reflecting walls,
 // receivers, materials and snapshots.
 //
-// Two kernel families compute the same arithmetic in the same order:
-//  * direct: one thread per updated cell, global loads only (the reference's
-//    decomposition of velocityKernel3D / stressKernel3D, cuda/Cuda_Seis3D.cu:222-310);
-//  * stream (2.5D): a 64 x BY workgroup owns an x-y tile and marches along z
-//    through a chunk of planes.  Each plane of every wavefield the half-step
-//    reads is staged once into LDS (x and/or y halo of 4) at the lead plane
-//    z+4; its x/y derivatives are filtered through CPML there and kept in a
-//    5-deep register queue, while the z-derivatives come from 9-deep register
-//    queues of the tile's own columns.  One pass over the wavefields per
-//    half-step; the blockIdx -> tile mapping keeps neighbouring tiles on one
-//    XCD so halos hit the same L2.
+// The direct kernel family (one thread per updated cell, global loads only: the
+// reference's decomposition of velocityKernel3D / stressKernel3D,
+// cuda/Cuda_Seis3D.cu:222-310) lives here; it computes the same arithmetic in the
+// same order as the LDS-DMA 2.5D family of seis3d_dma.hip, the default.
 // Built with -ffp-contract=off: every multiply and add rounds separately,
 // as in the FP32 CPU oracle.
 #include "seis3d_device.hpp"
@@ -141,324 +134,6 @@ __global__ __launch_bounds__(256) void stress_direct(DevStep P) {
     P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
     P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
     const bool bad = !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) || !isfinite(sxz);
-    if (bad) atomicOr(P.p.flags, 1u);
-}
-
-// ---------------------------------------------------------------------------
-// Stream (2.5D) family
-// ---------------------------------------------------------------------------
-
-// Cooperative load of a ROWS x COLS window of one plane into LDS (row
-// stride COLS).  Coordinates are clamped into the padded grid: clamped
-// values only feed lanes that store nothing.
-template <int ROWS, int COLS, int NT>
-__device__ __forceinline__ void stage(float* __restrict__ lds, const float* __restrict__ f, const DevGrid& g,
-                                      int xs, int ys, int z, int tid) {
-    const long long base = static_cast<long long>(z) * g.plane;
-#pragma unroll
-    for (int k0 = 0; k0 < ROWS * COLS; k0 += NT) {
-        const int k = k0 + tid;
-        if (ROWS * COLS % NT == 0 || k < ROWS * COLS) {
-            const int r = k / COLS, c = k - r * COLS;
-            const int xx = min(xs + c, g.gx - 1), yy = min(ys + r, g.gy - 1);
-            lds[k] = f[base + static_cast<long long>(yy) * g.px + xx];
-        }
-    }
-}
-
-
-// Fused velocity half-step, 2.5D.  Staged at the lead plane L = z+4:
-// sxx (x halo), sxy (x+y halo), syy (y halo), sxz (x halo), syz (y halo);
-// szz needs only its own column.
-template <int BY>
-__global__ __launch_bounds__(TX * BY) void velocity_stream(DevStep P, TileMap M) {
-    constexpr int NT = TX * BY;
-    constexpr int HY = BY + 8;
-    __shared__ float s_xx[BY * HX];
-    __shared__ float s_xy[HY * HX];
-    __shared__ float s_yy[HY * TX];
-    __shared__ float s_xz[BY * HX];
-    __shared__ float s_yz[HY * TX];
-
-    int bx, by, bz;
-    if (!tile_of(M, bx, by, bz)) return;
-    const DevGrid& g = P.g;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
-    const int x = x0 + tx, y = y0 + ty;
-    int z0, z1;
-    chunk_of(M, g.gz, bz, z0, z1);
-    const bool active = x < g.gx - 4 && y < g.gy - 4;
-    const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
-    const int qx = slab_q(x, g.gx, g.pml, false);
-    const int qy = slab_q(y, g.gy, g.pml, false);
-    const float invh = P.invh;
-
-    float q_xz[9], q_yz[9], q_zz[9];       // columns z-4..z+4
-    float p_vx[5], p_vy[5], p_vz[5];       // x+y filtered partial sums, planes z..z+4
-
-    const float* sxx = P.p.f[3];
-    const float* syy = P.p.f[4];
-    const float* szz = P.p.f[5];
-    const float* sxy = P.p.f[6];
-    const float* syz = P.p.f[7];
-    const float* sxz = P.p.f[8];
-
-    // Stage plane L, push its centres into the queues and, for an owned
-    // plane, the CPML-filtered x/y derivative sums into the partial queues.
-    auto stage_plane = [&](int L, bool owned) {
-        __syncthreads();
-        stage<BY, HX, NT>(s_xx, sxx, g, x0 - 4, y0, L, tid);
-        stage<HY, HX, NT>(s_xy, sxy, g, x0 - 4, y0 - 4, L, tid);
-        stage<HY, TX, NT>(s_yy, syy, g, x0, y0 - 4, L, tid);
-        stage<BY, HX, NT>(s_xz, sxz, g, x0 - 4, y0, L, tid);
-        stage<HY, TX, NT>(s_yz, syz, g, x0, y0 - 4, L, tid);
-        const float lead_zz = szz[cell(g, xc, yc, L)];
-        __syncthreads();
-        shift(q_xz, s_xz[ty * HX + tx + 4]);
-        shift(q_yz, s_yz[(ty + 4) * TX + tx]);
-        shift(q_zz, lead_zz);
-        float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
-        if (owned && active) {
-            float w[8];
-            // vx: D+x sxx (half), D-y sxy.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_xx[ty * HX + tx + 1 + j];
-            float d = stencil8(w, invh);
-            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 0, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d);
-            pvx += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_xy[(ty + j) * HX + tx + 4];
-            d = stencil8(w, invh);
-            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 0, qy, x, L), P.p.coef[1] + y * 3, d);
-            pvx += d;
-            // vy: D-x sxy, D+y syy (half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_xy[(ty + 4) * HX + tx + j];
-            d = stencil8(w, invh);
-            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 1, qx, y, L), P.p.coef[0] + x * 3, d);
-            pvy += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_yy[(ty + 1 + j) * TX + tx];
-            d = stencil8(w, invh);
-            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 1, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d);
-            pvy += d;
-            // vz: D-x sxz, D-y syz.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_xz[ty * HX + tx + j];
-            d = stencil8(w, invh);
-            if (qx >= 0) d = cpml_update(P.p.psi[0], psi_index_x(g, 2, qx, y, L), P.p.coef[0] + x * 3, d);
-            pvz += d;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_yz[(ty + j) * TX + tx];
-            d = stencil8(w, invh);
-            if (qy >= 0) d = cpml_update(P.p.psi[1], psi_index_y(g, 2, qy, x, L), P.p.coef[1] + y * 3, d);
-            pvz += d;
-        }
-        shift(p_vx, pvx);
-        shift(p_vy, pvy);
-        shift(p_vz, pvz);
-    };
-
-#pragma unroll
-    for (int j = 0; j < 9; ++j) { q_xz[j] = 0.0f; q_yz[j] = 0.0f; q_zz[j] = 0.0f; }
-#pragma unroll
-    for (int j = 0; j < 5; ++j) { p_vx[j] = 0.0f; p_vy[j] = 0.0f; p_vz[j] = 0.0f; }
-    // Prologue: columns z0-4..z0-1 (centres only), planes z0..z0+3 staged.
-    for (int L = z0 - 4; L < z0; ++L) {
-        shift(q_xz, sxz[cell(g, xc, yc, L)]);
-        shift(q_yz, syz[cell(g, xc, yc, L)]);
-        shift(q_zz, szz[cell(g, xc, yc, L)]);
-    }
-    for (int L = z0; L < z0 + 4; ++L) stage_plane(L, L < z1);
-
-    bool bad = false;
-    for (int z = z0; z < z1; ++z) {
-        stage_plane(z + 4, z + 4 < z1);
-        // Queues now hold columns z-4..z+4 and partials of planes z..z+4.
-        if (active) {
-            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
-            const long long i = cell(g, x, y, z);
-            const float* ib = P.p.inv_rho;
-            const float a = ib[i];
-            const float hx = face_buoyancy(a, ib[i + 1]);
-            const float hy = face_buoyancy(a, ib[i + g.px]);
-            const float hz = face_buoyancy(a, ib[i + g.plane]);
-            float w[8];
-            // vx: D-z sxz (columns z-4..z+3).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_xz[j];
-            float dz = stencil8(w, invh);
-            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 0, qz, x, y), P.p.coef[2] + z * 3, dz);
-            const float div_x = p_vx[0] + dz;
-            // vy: D-z syz.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_yz[j];
-            dz = stencil8(w, invh);
-            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 1, qz, x, y), P.p.coef[2] + z * 3, dz);
-            const float div_y = p_vy[0] + dz;
-            // vz: D+z szz (columns z-3..z+4, half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_zz[j + 1];
-            dz = stencil8(w, invh);
-            if (qz >= 0) dz = cpml_update(P.p.psi[2], psi_index_z(g, 2, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, dz);
-            const float div_z = p_vz[0] + dz;
-
-            float vx = P.p.f[0][i] + P.dt * hx * div_x;
-            float vy = P.p.f[1][i] + P.dt * hy * div_y;
-            float vz = P.p.f[2][i] + P.dt * hz * div_z;
-            if (P.src.active) {
-                const int r2 = source_r2(P.src, x, y, z);
-                if (r2 >= 0) {
-                    const float value = P.src.scale * P.src.w[r2] / P.src.norm;
-                    vx += value * P.src.dir[0] * hx;
-                    vy += value * P.src.dir[1] * hy;
-                    vz += value * P.src.dir[2] * hz;
-                }
-            }
-            P.p.f[0][i] = vx; P.p.f[1][i] = vy; P.p.f[2][i] = vz;
-            bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
-        }
-    }
-    if (bad) atomicOr(P.p.flags, 1u);
-}
-
-// Fused stress half-step, 2.5D.  vx, vy, vz staged at L = z+4 with x and y
-// halos; their six x/y derivatives are CPML-filtered at L and queued as
-// d00, d11, d01+d10, d20, d21.
-template <int BY>
-__global__ __launch_bounds__(TX * BY) void stress_stream(DevStep P, TileMap M) {
-    constexpr int NT = TX * BY;
-    constexpr int HY = BY + 8;
-    __shared__ float s_vx[HY * HX];
-    __shared__ float s_vy[HY * HX];
-    __shared__ float s_vz[HY * HX];
-
-    int bx, by, bz;
-    if (!tile_of(M, bx, by, bz)) return;
-    const DevGrid& g = P.g;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int x0 = 4 + bx * TX, y0 = 4 + by * BY;
-    const int x = x0 + tx, y = y0 + ty;
-    int z0, z1;
-    chunk_of(M, g.gz, bz, z0, z1);
-    const bool active = x < g.gx - 4 && y < g.gy - 4;
-    const int xc = min(x, g.gx - 1), yc = min(y, g.gy - 1);
-    const int qx = slab_q(x, g.gx, g.pml, false);
-    const int qy = slab_q(y, g.gy, g.pml, false);
-    const float invh = P.invh;
-
-    float q_vx[9], q_vy[9], q_vz[9];
-    float p00[5], p11[5], p01[5], p20[5], p21[5];
-
-    const float* fvx = P.p.f[0];
-    const float* fvy = P.p.f[1];
-    const float* fvz = P.p.f[2];
-
-    auto stage_plane = [&](int L, bool owned) {
-        __syncthreads();
-        stage<HY, HX, NT>(s_vx, fvx, g, x0 - 4, y0 - 4, L, tid);
-        stage<HY, HX, NT>(s_vy, fvy, g, x0 - 4, y0 - 4, L, tid);
-        stage<HY, HX, NT>(s_vz, fvz, g, x0 - 4, y0 - 4, L, tid);
-        __syncthreads();
-        const int c = (ty + 4) * HX + tx + 4;
-        shift(q_vx, s_vx[c]);
-        shift(q_vy, s_vy[c]);
-        shift(q_vz, s_vz[c]);
-        float d00 = 0.0f, d11 = 0.0f, s01 = 0.0f, d20 = 0.0f, d21 = 0.0f;
-        if (owned && active) {
-            float w[8];
-            // vx: D-x (slot 3, integer), D+y (slot 3, half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vx[(ty + 4) * HX + tx + j];
-            d00 = stencil8(w, invh);
-            if (qx >= 0) d00 = cpml_update(P.p.psi[0], psi_index_x(g, 3, qx, y, L), P.p.coef[0] + x * 3, d00);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vx[(ty + 1 + j) * HX + tx + 4];
-            float d01 = stencil8(w, invh);
-            if (qy >= 0) d01 = cpml_update(P.p.psi[1], psi_index_y(g, 3, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d01);
-            // vy: D+x (slot 4, half), D-y (slot 4, integer).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vy[(ty + 4) * HX + tx + 1 + j];
-            float d10 = stencil8(w, invh);
-            if (qx >= 0) d10 = cpml_update(P.p.psi[0], psi_index_x(g, 4, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d10);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vy[(ty + j) * HX + tx + 4];
-            d11 = stencil8(w, invh);
-            if (qy >= 0) d11 = cpml_update(P.p.psi[1], psi_index_y(g, 4, qy, x, L), P.p.coef[1] + y * 3, d11);
-            s01 = d01 + d10;
-            // vz: D+x (slot 5, half), D+y (slot 5, half).
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vz[(ty + 4) * HX + tx + 1 + j];
-            d20 = stencil8(w, invh);
-            if (qx >= 0) d20 = cpml_update(P.p.psi[0], psi_index_x(g, 5, qx, y, L), P.p.coef[0] + (g.gx + x) * 3, d20);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = s_vz[(ty + 1 + j) * HX + tx + 4];
-            d21 = stencil8(w, invh);
-            if (qy >= 0) d21 = cpml_update(P.p.psi[1], psi_index_y(g, 5, qy, x, L), P.p.coef[1] + (g.gy + y) * 3, d21);
-        }
-        shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
-    };
-
-#pragma unroll
-    for (int j = 0; j < 9; ++j) { q_vx[j] = 0.0f; q_vy[j] = 0.0f; q_vz[j] = 0.0f; }
-#pragma unroll
-    for (int j = 0; j < 5; ++j) { p00[j] = 0.0f; p11[j] = 0.0f; p01[j] = 0.0f; p20[j] = 0.0f; p21[j] = 0.0f; }
-    for (int L = z0 - 4; L < z0; ++L) {
-        shift(q_vx, fvx[cell(g, xc, yc, L)]);
-        shift(q_vy, fvy[cell(g, xc, yc, L)]);
-        shift(q_vz, fvz[cell(g, xc, yc, L)]);
-    }
-    for (int L = z0; L < z0 + 4; ++L) stage_plane(L, L < z1);
-
-    bool bad = false;
-    for (int z = z0; z < z1; ++z) {
-        stage_plane(z + 4, z + 4 < z1);
-        if (active) {
-            const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
-            const long long i = cell(g, x, y, z);
-            float w[8];
-            // D+z vx, D+z vy: columns z-3..z+4 (half); D-z vz: z-4..z+3.
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_vx[j + 1];
-            float d02 = stencil8(w, invh);
-            if (qz >= 0) d02 = cpml_update(P.p.psi[2], psi_index_z(g, 3, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, d02);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_vy[j + 1];
-            float d12 = stencil8(w, invh);
-            if (qz >= 0) d12 = cpml_update(P.p.psi[2], psi_index_z(g, 4, qz, x, y), P.p.coef[2] + (g.gz + z) * 3, d12);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = q_vz[j];
-            float d22 = stencil8(w, invh);
-            if (qz >= 0) d22 = cpml_update(P.p.psi[2], psi_index_z(g, 5, qz, x, y), P.p.coef[2] + z * 3, d22);
-
-            const float* m = P.p.mu;
-            const float lam = P.p.lambda[i], mu = m[i];
-            const bool surface = g.free_surface && z == 4;
-            const float d00 = p00[0], d11 = p11[0];
-            if (surface) d22 = -lam / (lam + 2 * mu) * (d00 + d11);
-            const float trace = d00 + d11 + d22;
-            const long long sx = 1, sy = g.px, sz = g.plane;
-            float sxx = P.p.f[3][i] + P.dt * (lam * trace + 2 * mu * d00);
-            float syy = P.p.f[4][i] + P.dt * (lam * trace + 2 * mu * d11);
-            float szz = surface ? 0.0f : P.p.f[5][i] + P.dt * (lam * trace + 2 * mu * d22);
-            const float* r = P.p.inv_mu;
-            const float r0 = r[i], rx = r[i + sx], ry = r[i + sy], rz = r[i + sz];
-            const float sxy = P.p.f[6][i] + P.dt * shear_from_inverse(r0, rx, ry, r[i + sx + sy]) * p01[0];
-            const float syz = P.p.f[7][i] + P.dt * shear_from_inverse(r0, ry, rz, r[i + sy + sz]) * (d12 + p21[0]);
-            const float sxz = P.p.f[8][i] + P.dt * shear_from_inverse(r0, rx, rz, r[i + sx + sz]) * (d02 + p20[0]);
-            if (P.src.active) {
-                const int r2 = source_r2(P.src, x, y, z);
-                if (r2 >= 0) {
-                    const float value = P.src.scale * P.src.w[r2] / P.src.norm;
-                    sxx += value; syy += value; szz += value;
-                }
-            }
-            P.p.f[3][i] = sxx; P.p.f[4][i] = syy; P.p.f[5][i] = szz;
-            P.p.f[6][i] = sxy; P.p.f[7][i] = syz; P.p.f[8][i] = sxz;
-            bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) || !isfinite(sxz);
-        }
-    }
     if (bad) atomicOr(P.p.flags, 1u);
 }
 
@@ -744,41 +419,9 @@ __global__ __launch_bounds__(256) void extract_views(DevStep P, ViewParams v, lo
 // Launchers
 // ---------------------------------------------------------------------------
 
-static hipError_t stream_launch(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {
-    TileMap m;
-    m.nbx = (P.g.gx - 8 + TX - 1) / TX;
-    m.nby = (P.g.gy - 8 + tile_y - 1) / tile_y;
-    m.zchunk = zchunk;
-    m.nbz = (P.g.gz - 8 + zchunk - 1) / zchunk;
-    m.nblocks = m.nbx * m.nby * m.nbz;
-    const int launched = (m.nblocks + 7) / 8 * 8;
-    m.per_xcd = launched / 8;
-    const dim3 grid(launched);
-    switch (tile_y) {
-    case 4:
-        if (velocity) hipLaunchKernelGGL(velocity_stream<4>, grid, dim3(TX, 4), 0, s, P, m);
-        else hipLaunchKernelGGL(stress_stream<4>, grid, dim3(TX, 4), 0, s, P, m);
-        break;
-    case 8:
-        if (velocity) hipLaunchKernelGGL(velocity_stream<8>, grid, dim3(TX, 8), 0, s, P, m);
-        else hipLaunchKernelGGL(stress_stream<8>, grid, dim3(TX, 8), 0, s, P, m);
-        break;
-    case 16:
-        if (velocity) hipLaunchKernelGGL(velocity_stream<16>, grid, dim3(TX, 16), 0, s, P, m);
-        else hipLaunchKernelGGL(stress_stream<16>, grid, dim3(TX, 16), 0, s, P, m);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s) {
     if (k.family == 4) return launch_dma(true, P, k.tile_y, k.z_chunk, s);
     if (out_of_place(P.p)) return hipErrorInvalidValue;   // in-place families only
-    if (k.family == 3) return launch_column(true, P, k.tile_y, k.z_chunk, s);
-    if (k.family == 2) return launch_pipe(true, P, k.tile_y, k.z_chunk, s);
-    if (k.family == 1) return stream_launch(true, P, k.tile_y, k.z_chunk, s);
     const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
     hipLaunchKernelGGL(velocity_direct, grid, block, 0, s, P);
     return hipGetLastError();
@@ -787,9 +430,6 @@ hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t 
 hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s) {
     if (k.family == 4) return launch_dma(false, P, k.tile_y, k.z_chunk, s);
     if (out_of_place(P.p)) return hipErrorInvalidValue;
-    if (k.family == 3) return launch_column(false, P, k.tile_y, k.z_chunk, s);
-    if (k.family == 2) return launch_pipe(false, P, k.tile_y, k.z_chunk, s);
-    if (k.family == 1) return stream_launch(false, P, k.tile_y, k.z_chunk, s);
     const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4, P.g.gz - 8);
     hipLaunchKernelGGL(stress_direct, grid, block, 0, s, P);
     return hipGetLastError();

@@ -9,9 +9,9 @@ namespace seis3d {
 
 // Kernel family and tile shape of the two fused half-steps.
 struct KernelConfig {
-    int family = 2;    // 0 direct, 1 stream (2.5D), 2 pipelined stream (2.5D), 3 column march, 4 LDS-DMA stream (2.5D)
-    int tile_y = 8;    // rows per workgroup: 4, 8 or 16 (family 1), 4 or 8 (family 2)
-    int z_chunk = 32;  // planes per workgroup (stream)
+    int family = 4;    // 0 direct (one thread per cell), 4 LDS-DMA 2.5D stream
+    int tile_y = 4;    // rows per workgroup of family 4: 4 or 8
+    int z_chunk = 16;  // planes per workgroup of family 4
 };
 
 // Procedural model parameters of generateMaterialsKernel3D.
@@ -46,10 +46,6 @@ hipError_t launch_extract_views(const DevStep& P, const ViewParams& vp, long lon
                                 hipStream_t s);
 
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s);
-// Software-pipelined 2.5D family (seis3d_pipe.hip), tile_y 4 or 8.
-hipError_t launch_pipe(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
-// Column-marching family (seis3d_column.hip), tile_y 1, 2, 4 or 8.
-hipError_t launch_column(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
 // LDS-DMA staged 2.5D family (seis3d_dma.hip), tile_y 4 or 8.
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
 // Resident workgroups of the family-4 pair per CU (the smaller of velocity and

@@ -163,6 +163,7 @@ def library() -> C.CDLL:
             "seis3d_gather_timing": (C.c_int, [P(Ctx), C.c_int, P(C.c_int), P(C.c_double)]),
             "seis3d_ricker": (C.c_float, [C.c_float, C.c_double]),
             "seis3d_create": (C.c_int, [P(Ctx), P(vp)]),
+            "seis3d_bandwidth_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(C.c_double), P(C.c_double)]),
             "seis3d_destroy": (None, [vp]),
             "seis3d_get_ctx": (C.c_int, [vp, P(Ctx)]),
             "seis3d_reset": (C.c_int, [vp]),
@@ -362,6 +363,16 @@ def ricker(frequency: float, time: float) -> float:
     return float(library().seis3d_ricker(frequency, time))
 
 
+PROBE_KINDS = {"read": 0, "write": 1, "copy": 2, "copy_lds_dma": 3, "mix10_3": 4, "mix12_6": 5}
+
+
+def bandwidth_probe(kind: str, mib: int = 1024, reps: int = 10) -> dict:
+    """Project-written HBM stream kernels (the measured ceiling): GB/s and microseconds per launch."""
+    gbs, us = C.c_double(), C.c_double()
+    _check(library().seis3d_bandwidth_probe(PROBE_KINDS[kind], int(mib) << 20, int(reps), C.byref(gbs), C.byref(us)))
+    return {"kind": kind, "mib_per_array": mib, "gbs": gbs.value, "us_per_launch": us.value}
+
+
 def version() -> str:
     return library().seis3d_version().decode()
 
@@ -536,8 +547,9 @@ class GPUSimData3D:
         out = np.zeros(2 * n, np.float32)
         sl = (C.c_int32 * 3)(*[int(v) for v in slices])
         st = ViewStats()
-        _check(library().seis3d_extract_views(self._h, _fptr(out), out.size, sl, int(component), int(material_component),
-                                              int(volume), int(include_boundary), C.byref(st)))
+        fn = library().seis3d_extract_views
+        _check(fn(self._h, _fptr(out), out.size, sl, int(component), int(material_component), int(volume),
+                  int(include_boundary), C.byref(st)))
         return out.reshape(n, 2), {"maximum": st.maximum, "material_range": (st.material_min, st.material_max),
                                    "velocity_maximum": st.velocity_maximum}
 

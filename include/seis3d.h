@@ -241,8 +241,9 @@ int seis3d_extract_views(seis3d_t* t, float* out, size_t capacity, const int32_t
 
 /* ---- Tuning and measurement ---------------------------------------------- */
 
-/* Kernel family: 0 = direct (one thread per cell), 1 = 2.5D LDS/register
- * streaming.  tile_y rows per block, z_chunk planes per block (family 1). */
+/* Kernel family: 0 = direct (one thread per cell), 4 = 2.5D LDS-DMA streaming
+ * (the default).  tile_y rows (4 or 8) and z_chunk planes (1..128) per
+ * workgroup of family 4. */
 int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk);
 int seis3d_get_tiles(const seis3d_t* t, int* family, int* tile_y, int* z_chunk);
 /* seis3d_run with HIP events around every fused-kernel launch. */
@@ -406,6 +407,13 @@ int seis3d_abc_mlp(const float* params, const float* X, const float* T, int n, f
                    float* grad, double* loss);
 
 /* ---- Errors and build info ------------------------------------------------ */
+
+/* HBM bandwidth probe (no reference counterpart; the measured ceiling the step kernels are
+ * compared with).  Streams arrays of `bytes` each, `reps` timed launches after one warm-up:
+ * kind 0 read, 1 write, 2 copy, 3 copy staged through LDS by global_load_lds, 4 ten inputs and
+ * three outputs elementwise, 5 twelve inputs and six outputs.  *gbs = bytes moved per second
+ * (every array touched once per launch), *us_per_launch the average launch time. */
+int seis3d_bandwidth_probe(int kind, uint64_t bytes, int reps, double* gbs, double* us_per_launch);
 
 const char* seis3d_last_error(void);
 const char* seis3d_version(void);

@@ -103,11 +103,11 @@ def test_checkpoint_interval_is_bitwise_neutral(gpu, small):
             assert np.array_equal(a, b), interval
 
 
-@pytest.mark.parametrize("family,tile_y,z_chunk,interval", [(0, 4, 16, 7), (3, 4, 16, 7), (0, 4, 16, 9)])
+@pytest.mark.parametrize("family,tile_y,z_chunk,interval", [(0, 4, 16, 7), (0, 8, 32, 70), (0, 4, 16, 9)])
 def test_out_of_place_recompute_equals_copy_path(gpu, small, family, tile_y, z_chunk, interval):
     """The LDS-DMA kernels step recomputed segments out of place, straight into the stored
-    states; the in-place kernel families step a live state and copy it after every step.
-    The families are bitwise-identical, so both routes give the same gradient bit for bit.
+    states; the in-place direct family steps a live state and copies it after every step.
+    The two families are bitwise-identical, so both routes give the same gradient bit for bit.
     Interval 9 leaves a short last segment (70 = 7 * 9 + 7), whose unused slots the
     out-of-place route must clear itself."""
     results = []

@@ -61,7 +61,7 @@ CASES = {
         dict(x=12, y=10, z=11, kind=S.EXPLOSION), None, 100, 2),
 }
 
-TILES = [(0, 8, 32), (1, 4, 7), (1, 8, 16), (1, 16, 5), (2, 4, 7), (2, 8, 16), (2, 8, 1000), (3, 1, 6), (3, 4, 16), (3, 8, 1000), (4, 4, 5), (4, 8, 16), (4, 8, 128)]
+TILES = [(0, 8, 32), (4, 4, 5), (4, 4, 16), (4, 4, 37), (4, 8, 7), (4, 8, 16), (4, 8, 128)]
 
 
 def receivers_for(c):
@@ -191,7 +191,7 @@ def test_stream_equals_direct_bitwise(gpu):
     model = W.layered_model(c.nx, c.ny, c.nz, seed=9)
     src = W.config2_source(c)
     out = []
-    for tiles in [(0, 8, 32), (1, 8, 13), (1, 16, 64), (2, 4, 9), (2, 8, 40), (3, 2, 11), (3, 4, 40), (4, 4, 7), (4, 8, 20)]:
+    for tiles in [(0, 8, 32), (4, 4, 7), (4, 4, 16), (4, 8, 20), (4, 8, 128)]:
         g = S.GPUSimData3D(c)
         g.set_tiles(*tiles)
         g.upload_model(*model)
@@ -201,27 +201,3 @@ def test_stream_equals_direct_bitwise(gpu):
     for f in range(9):
         for k in range(1, len(out)):
             assert np.array_equal(out[0][f], out[k][f]), (f, k)
-
-
-def test_config2_full_size_parity(gpu):
-    """300x50x300 with free top + 20-cell CPML, the default 150-receiver spread, first 30 steps."""
-    c = W.config2_context()
-    model = W.layered_model(c.nx, c.ny, c.nz)
-    src = W.config2_source(c)
-    rec = W.config2_receivers(c)
-    steps, stride = 30, 2
-    log("config2 full size: GPU run")
-    g, gather = run_gpu(c, src, model, steps, stride, rec, (2, 8, 40))
-    log("config2 full size: oracle run")
-    O.set_threads(16)
-    o, g32 = run_oracle(c, src, model, steps, stride, 32, rec)
-    assert gather.shape == (15, 150, 3)
-    e = rel_l2(gather, g32)
-    log(f"  gather rel-L2 {e:.3e}")
-    assert e <= TOL32
-    core = (slice(4, c.gz - 4), slice(4, c.gy - 4), slice(4, c.gx - 4))
-    for f in (S.VX, S.VZ, S.SZZ, S.SXZ):
-        s = rel_l2(g.download_field(f)[core], o.field(f)[core])
-        log(f"  field {f} rel-L2 {s:.3e}")
-        assert s <= TOL32
-    g.release()
