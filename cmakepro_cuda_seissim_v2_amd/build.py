@@ -34,7 +34,7 @@ HIP_FLAGS = [
 LIBRARIES = {
     "libseis2d.so": ["seis2d_kernels.hip", "seis2d_api.cpp", "seis2d_io.cpp"],
     "libseis3d.so": ["seis3d_kernels.hip", "seis3d_dma.hip", "seis3d_adjoint.hip",
-                    "seis3d_abc.hip", "seis3d_stream.hip", "seis3d_api.cpp", "seis3d_fwi.cpp", "seis3d_abc.cpp", "seis3d_io.cpp"],
+                    "seis3d_abc.hip", "seis3d_stream.hip", "seis3d_noalloc.hip", "seis3d_api.cpp", "seis3d_fwi.cpp", "seis3d_abc.cpp", "seis3d_io.cpp"],
 }
 
 

@@ -190,6 +190,8 @@ def library() -> C.CDLL:
             "seis3d_make_view_layout": (C.c_int, [P(Ctx), P(ViewLayout)]),
             "seis3d_extract_views": (C.c_int, [vp, P(C.c_float), C.c_size_t, P(C.c_int32), C.c_int, C.c_int, C.c_int,
                                                C.c_int, P(ViewStats)]),
+            "seis3d_extract_views_noalloc": (C.c_int, [vp, P(C.c_float), C.c_size_t, P(C.c_int32), C.c_int, C.c_int,
+                                                       C.c_int, C.c_int, P(ViewStats)]),
             "seis3d_get_tiles": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
             "seis3d_run_timed": (C.c_int, [vp, P(Source), C.c_uint64, C.c_int64, P(Timing)]),
             "seis3d_step_bytes": (C.c_int, [P(Ctx), P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
@@ -540,14 +542,15 @@ class GPUSimData3D:
         return out.reshape(dom[v], dom[u])
 
     def extract_views(self, slices, component: int = 0, material_component: int = 5, volume: bool = True,
-                      include_boundary: bool = False):
-        """extractViews3D: ([sample][wave, material] array, statistics dict)."""
+                      include_boundary: bool = False, no_alloc: bool = False):
+        """extractViews3D: ([sample][wave, material] array, statistics dict).  no_alloc: through
+        seis3d_extract_views_noalloc (the library's fixed per-GPU staging; no device allocation)."""
         lay = make_view_layout(self.ctx)
         n = lay.total_samples if volume else lay.volume_offset
         out = np.zeros(2 * n, np.float32)
         sl = (C.c_int32 * 3)(*[int(v) for v in slices])
         st = ViewStats()
-        fn = library().seis3d_extract_views
+        fn = library().seis3d_extract_views_noalloc if no_alloc else library().seis3d_extract_views
         _check(fn(self._h, _fptr(out), out.size, sl, int(component), int(material_component), int(volume),
                   int(include_boundary), C.byref(st)))
         return out.reshape(n, 2), {"maximum": st.maximum, "material_range": (st.material_min, st.material_max),

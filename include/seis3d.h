@@ -238,6 +238,16 @@ typedef struct seis3d_view_stats {
  * non-finite sample fails with code 4. */
 int seis3d_extract_views(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
                          int material_component, int volume, int include_boundary, seis3d_view_stats* stats);
+/* No reference counterpart: seis3d_extract_views for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors, but it calls no hipMalloc/hipFree: the device staging is
+ * a fixed array of the library, one per GPU (16 MiB + 256 B, held until the process ends), so
+ * at most 2,097,152 samples per call (the three planes alone always fit; a volume may not);
+ * a larger request fails with code 1.  Calls that overlap on one GPU share that array and race.
+ * The session's device must be current. */
+int seis3d_extract_views_noalloc(seis3d_t* t, float* out, size_t capacity, const int32_t slices[3], int component,
+                                 int material_component, int volume, int include_boundary,
+                                 seis3d_view_stats* stats);
 
 /* ---- Tuning and measurement ---------------------------------------------- */
 
