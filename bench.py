@@ -141,20 +141,27 @@ def batch_bench(args) -> None:
     t_plain = time.perf_counter() - t0
     print(f"bench: plain loop {t_plain:.3f} s", file=sys.stderr, flush=True)
 
-    # Batched graph.
-    for g in sessions:
-        g.reset()
-        g.gather_begin(steps // stride, stride)
-    batch = S.Batch(sessions, [s[2] for s in shots], steps)
-    for g in sessions:
-        g.synchronize()
-    t0 = time.perf_counter()
-    batch.launch()
-    batch.synchronize()
-    t_batch = time.perf_counter() - t0
-    same = all(np.array_equal(g.gather_read(), p) for g, p in zip(sessions, plain))
-    print(f"bench: batched graph {t_batch:.3f} s, gathers identical to plain: {same}", file=sys.stderr, flush=True)
-    batch.close()
+    # Batched graphs: every shot on its own stream (the default), and shared launches (groups of
+    # shots stepped by one launch per half-step).
+    times, same = {}, True
+    for mode in ("shot_streams", "shared_launches"):
+        for g in sessions:
+            g.reset()
+            g.gather_end()
+            g.gather_begin(steps // stride, stride)
+        batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=mode == "shared_launches")
+        for g in sessions:
+            g.synchronize()
+        t0 = time.perf_counter()
+        batch.launch()
+        batch.synchronize()
+        times[mode] = time.perf_counter() - t0
+        ok = all(np.array_equal(g.gather_read(), p) for g, p in zip(sessions, plain))
+        same = same and ok
+        print(f"bench: batched graph ({mode}) {times[mode]:.3f} s, gathers identical to plain: {ok}",
+              file=sys.stderr, flush=True)
+        batch.close()
+    t_batch = times["shot_streams"]
     for g in sessions:
         g.release()
     out = {
@@ -171,6 +178,8 @@ def batch_bench(args) -> None:
             "updated_cells_total": n_upd,
             "plain_shots_per_s": round(len(shots) / t_plain, 4),
             "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "shared_launches_shots_per_s": round(len(shots) / times["shared_launches"], 4),
+            "shared_launches_speedup_vs_plain": round(t_plain / times["shared_launches"], 3),
             "gathers_bit_identical": bool(same),
             "gcell_updates_per_s_batched": round(n_upd * steps / t_batch / 1e9, 3),
             "autotuned": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in tuned.items()},

@@ -1119,6 +1119,13 @@ struct seis3d_batch {
     std::vector<hipEvent_t> events;
     int64_t steps = 0;
     bool holds = false;   // sessions' batch_refs were raised by this batch
+    bool multi = false;   // one launch per half-step for every shot (else a stream per shot)
+    // Device tables of the multi-shot ghost and receiver launches: per step and half-step every
+    // shot's step parameters, every shot's spacing, and the receiver samples.  (The half-steps
+    // themselves take their shots' parameters as kernel arguments.)
+    seis3d::DevStep* d_steps = nullptr;
+    float* d_spacing = nullptr;
+    seis3d::ShotSample* d_samples = nullptr;
 
     void release() noexcept {
         if (origin) (void)hipStreamSynchronize(origin);
@@ -1128,6 +1135,8 @@ struct seis3d_batch {
         if (origin) (void)hipStreamDestroy(origin);
         exec = nullptr; graph = nullptr; origin = nullptr;
         events.clear();
+        (void)hipFree(d_steps); (void)hipFree(d_spacing); (void)hipFree(d_samples);
+        d_steps = nullptr; d_spacing = nullptr; d_samples = nullptr;
         if (holds) {
             for (auto* t : sessions) {
                 if (--t->batch_refs == 0 && t->destroy_pending) {
@@ -1196,9 +1205,166 @@ int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* famil
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Shots with CPML or a free top (no reflecting walls, no learned frame) on one device step in
+// multi-shot launches: each half-step of every shot is one launch per tile height.
+bool multi_shot_ok(const std::vector<seis3d_t*>& sessions) {
+    for (auto* t : sessions)
+        if (t->c.reflecting || t->abc || t->device != sessions[0]->device) return false;
+    return true;
+}
+
+// The LDS-DMA shape a shot steps with in a multi-shot launch: its own when it uses that family;
+// the default shape for a shot on the direct family (the two families round identically).
+seis3d::KernelConfig multi_shot_shape(const seis3d_t* t) {
+    seis3d::KernelConfig k = t->kernels;
+    if (k.family != 4 || (k.tile_y != 4 && k.tile_y != 8) || k.z_chunk < 1 || k.z_chunk > 128) {
+        k.family = 4;
+        k.tile_y = 4;
+        k.z_chunk = std::min(16, std::max(1, t->c.gz - 8));
+    }
+    return k;
+}
+
+// Builds the multi-shot tables and captures every step of every shot.  Shots are grouped by
+// tile height, at most kShotsPerLaunch per group; each group steps with one launch per
+// half-step on its own stream (the stream of its first session), forked from and joined to
+// b->origin, so that different groups' velocity and stress launches overlap.  The gather
+// bookkeeping advances as maybe_sample would.
+void capture_multi_shot(seis3d_batch* b, const seis3d_source* sources, int64_t n_steps) {
+    const int S = int(b->sessions.size());
+    std::vector<int> order(S);
+    for (int s = 0; s < S; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return multi_shot_shape(b->sessions[a]).tile_y < multi_shot_shape(b->sessions[c]).tile_y;
+    });
+    struct Group {
+        int begin, end, tile_y;
+        int max_gx, max_gy, max_rec;
+        bool any_free;
+        std::vector<std::size_t> start;   // per step: first sample entry of this group
+    };
+    std::vector<Group> groups;
+    std::vector<seis3d::DevStep> table(std::size_t(2 * n_steps) * S);
+    std::vector<seis3d::TileMap> maps(S);
+    std::vector<float> spacing(S);
+    for (int j = 0; j < S; ++j) {
+        seis3d_t* t = b->sessions[order[j]];
+        const seis3d::KernelConfig k = multi_shot_shape(t);
+        if (groups.empty() || groups.back().tile_y != k.tile_y ||
+            groups.back().end - groups.back().begin == seis3d::kShotsPerLaunch)
+            groups.push_back({j, j, k.tile_y, 8, 8, 0, false, {}});
+        Group& gr = groups.back();
+        gr.end = j + 1;
+        gr.max_gx = std::max(gr.max_gx, t->c.gx);
+        gr.max_gy = std::max(gr.max_gy, t->c.gy);
+        gr.max_rec = std::max(gr.max_rec, t->receiver_count);
+        gr.any_free = gr.any_free || t->c.free_surface;
+        maps[j] = seis3d::dma_tile_map(t->base().g, k.tile_y, k.z_chunk);
+        spacing[j] = t->c.spacing;
+    }
+    // The kernel argument of one group's launch for table row `row` (steps of one half-step).
+    auto launch_args = [&](const Group& gr, const seis3d::DevStep* row) {
+        seis3d::ShotLaunch L{};
+        L.nshots = gr.end - gr.begin;
+        for (int i = 0; i < L.nshots; ++i) {
+            L.maps[i] = maps[gr.begin + i];
+            L.steps[i] = row[gr.begin + i];
+            L.first[i + 1] = L.first[i] + L.maps[i].nblocks;
+        }
+        L.nblocks = L.first[L.nshots];
+        L.per_xcd = (L.nblocks + 7) / 8;
+        return L;
+    };
+    for (int64_t n = 0; n < n_steps; ++n) {
+        for (int j = 0; j < S; ++j) {
+            const int s = order[j];
+            seis3d_t* t = b->sessions[s];
+            const seis3d_ctx& c = t->c;
+            seis3d::DevStep V = t->base(), St = V;
+            if (sources[s].kind == SEIS3D_FORCE) V.src = make_source(c, sources[s], double(n) * c.dt);
+            if (sources[s].kind == SEIS3D_EXPLOSION) St.src = make_source(c, sources[s], (double(n) + 0.5) * c.dt);
+            table[std::size_t(2 * n) * S + j] = V;
+            table[std::size_t(2 * n + 1) * S + j] = St;
+        }
+    }
+    // Receiver samples, per group and step; entries index the group's rows of the step table.
+    std::vector<seis3d::ShotSample> samples;
+    for (auto& gr : groups) {
+        gr.start.assign(std::size_t(n_steps) + 1, samples.size());
+        for (int64_t n = 0; n < n_steps; ++n) {
+            for (int j = gr.begin; j < gr.end; ++j) {
+                seis3d_t* t = b->sessions[order[j]];
+                if (t->recording && t->gather_count < t->gather_target &&
+                    (std::uint64_t(n) + 1) % t->gather_stride == 0) {
+                    samples.push_back({j - gr.begin, t->receiver_count, t->receivers,
+                                       t->gather + std::size_t(t->gather_count) * t->gather_receivers * 3});
+                    if (++t->gather_count >= t->gather_target) t->recording = false;
+                }
+            }
+            gr.start[std::size_t(n) + 1] = samples.size();
+        }
+    }
+    check(hipMalloc(&b->d_steps, table.size() * sizeof(seis3d::DevStep)), "allocate batch tables");
+    check(hipMalloc(&b->d_spacing, spacing.size() * sizeof(float)), "allocate batch tables");
+    if (!samples.empty())
+        check(hipMalloc(&b->d_samples, samples.size() * sizeof(seis3d::ShotSample)), "allocate batch tables");
+    // The tables go up before capture starts (a synchronous copy may not be captured).
+    check(hipMemcpy(b->d_steps, table.data(), table.size() * sizeof(seis3d::DevStep), hipMemcpyHostToDevice),
+          "upload batch tables");
+    check(hipMemcpy(b->d_spacing, spacing.data(), spacing.size() * sizeof(float), hipMemcpyHostToDevice),
+          "upload batch tables");
+    if (!samples.empty())
+        check(hipMemcpy(b->d_samples, samples.data(), samples.size() * sizeof(seis3d::ShotSample),
+                        hipMemcpyHostToDevice),
+              "upload batch tables");
+    check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "begin capture");
+    check(hipEventRecord(b->events[0], b->origin), "fork capture");
+    for (std::size_t q = 0; q < groups.size(); ++q) {
+        const Group& gr = groups[q];
+        hipStream_t st = b->sessions[order[gr.begin]]->stream;
+        check(hipStreamWaitEvent(st, b->events[0], 0), "fork capture");
+        for (int j = gr.begin; j < gr.end; ++j) {
+            seis3d_t* t = b->sessions[order[j]];
+            if (t->c.free_surface) check(seis3d::launch_stress_ghost(t->base(), st), "kernel launch");
+        }
+        for (int64_t n = 0; n < n_steps; ++n) {
+            const seis3d::DevStep* hv = table.data() + std::size_t(2 * n) * S;
+            const seis3d::DevStep* vel = b->d_steps + std::size_t(2 * n) * S + gr.begin;
+            check(seis3d::launch_dma_shots(true, launch_args(gr, hv), gr.tile_y, st), "kernel launch");
+            if (gr.any_free)
+                check(seis3d::launch_velocity_ghost_shots(vel, b->d_spacing + gr.begin, gr.end - gr.begin, gr.max_gx,
+                                                          gr.max_gy, st),
+                      "kernel launch");
+            check(seis3d::launch_dma_shots(false, launch_args(gr, hv + S), gr.tile_y, st), "kernel launch");
+            const std::size_t k = gr.start[std::size_t(n) + 1] - gr.start[std::size_t(n)];
+            if (k)
+                check(seis3d::launch_receivers_shots(vel, b->d_samples + gr.start[std::size_t(n)], int(k), gr.max_rec, st),
+                      "kernel launch");
+        }
+        check(hipEventRecord(b->events[q + 1], st), "join capture");
+        check(hipStreamWaitEvent(b->origin, b->events[q + 1], 0), "join capture");
+    }
+    check(hipStreamEndCapture(b->origin, &b->graph), "end capture");
+}
+
+}  // namespace
+
+extern "C" {
+
 int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources, int count, int64_t n_steps,
                         seis3d_batch_t** out) {
+    return seis3d_batch_create_mode(sessions, sources, count, n_steps, SEIS3D_BATCH_SHOT_STREAMS, out);
+}
+
+int seis3d_batch_create_mode(seis3d_t* const* sessions, const seis3d_source* sources, int count, int64_t n_steps,
+                             int mode, seis3d_batch_t** out) {
     return guarded([&] {
+        if (mode != SEIS3D_BATCH_SHOT_STREAMS && mode != SEIS3D_BATCH_SHARED_LAUNCHES)
+            invalid("Seis3D: unknown batch mode.");
         if (!sessions || !sources || !out || count < 1 || count > 256 || n_steps < 1 || n_steps > 1000000)
             invalid("Seis3D: invalid batch request.");
         for (int s = 0; s < count; ++s) {
@@ -1227,26 +1393,36 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
                 check(hipStreamSynchronize(t->stream), "batch capture");
                 b->before[std::size_t(s)] = {t->gather_count, t->recording, t->ghosts_valid};
             }
-            // Every session's stream joins one capture: fork from the origin stream,
-            // enqueue all steps of every shot on its own stream, join back.  Each shot
-            // starts by rebuilding its stress ghost planes, so a launch is valid whatever
-            // was written into the wavefield since the last one.
-            check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "begin capture");
-            check(hipEventRecord(b->events[0], b->origin), "fork capture");
-            for (int s = 0; s < count; ++s) {
-                seis3d_t* t = b->sessions[s];
-                check(hipStreamWaitEvent(t->stream, b->events[0], 0), "fork capture");
-                if (t->c.free_surface) check(seis3d::launch_stress_ghost(t->base(), t->stream), "kernel launch");
-                t->ghosts_valid = true;
-                for (int64_t k = 0; k < n_steps; ++k) {
-                    enqueue_step(t, sources[s], std::uint64_t(k));
-                    maybe_sample(t, std::uint64_t(k));
+            // Each shot starts by rebuilding its stress ghost planes, so a launch is valid
+            // whatever was written into the wavefield since the last one.
+            b->multi = mode == SEIS3D_BATCH_SHARED_LAUNCHES && multi_shot_ok(b->sessions);
+            if (b->multi) {
+                // One launch per half-step for all shots: the shots' tiles share the GPU.
+                capture_multi_shot(b, sources, n_steps);
+                for (int s = 0; s < count; ++s) {
+                    seis3d_t* t = b->sessions[s];
+                    b->after[std::size_t(s)] = {t->gather_count, t->recording, true};
                 }
-                b->after[std::size_t(s)] = {t->gather_count, t->recording, true};
-                check(hipEventRecord(b->events[s + 1], t->stream), "join capture");
-                check(hipStreamWaitEvent(b->origin, b->events[s + 1], 0), "join capture");
+            } else {
+                // Every session's stream joins one capture: fork from the origin stream,
+                // enqueue all steps of every shot on its own stream, join back.
+                check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "begin capture");
+                check(hipEventRecord(b->events[0], b->origin), "fork capture");
+                for (int s = 0; s < count; ++s) {
+                    seis3d_t* t = b->sessions[s];
+                    check(hipStreamWaitEvent(t->stream, b->events[0], 0), "fork capture");
+                    if (t->c.free_surface) check(seis3d::launch_stress_ghost(t->base(), t->stream), "kernel launch");
+                    t->ghosts_valid = true;
+                    for (int64_t k = 0; k < n_steps; ++k) {
+                        enqueue_step(t, sources[s], std::uint64_t(k));
+                        maybe_sample(t, std::uint64_t(k));
+                    }
+                    b->after[std::size_t(s)] = {t->gather_count, t->recording, true};
+                    check(hipEventRecord(b->events[s + 1], t->stream), "join capture");
+                    check(hipStreamWaitEvent(b->origin, b->events[s + 1], 0), "join capture");
+                }
+                check(hipStreamEndCapture(b->origin, &b->graph), "end capture");
             }
-            check(hipStreamEndCapture(b->origin, &b->graph), "end capture");
             check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate graph");
         } catch (...) {
             if (b->origin) {

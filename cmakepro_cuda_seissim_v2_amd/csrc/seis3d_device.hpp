@@ -178,4 +178,19 @@ __device__ __forceinline__ void shift(float (&q)[N], float lead) {
     q[N - 1] = lead;
 }
 
+// Several shots' tiles in one launch, passed by value as the kernel argument, so that every
+// shot's parameters are read from the kernel-argument segment exactly as a single shot's are
+// (constant memory: its field pointers are known to address global memory).  Shot s owns
+// logical tiles [first[s], first[s + 1]).
+constexpr int kShotsPerLaunch = 4;
+struct ShotLaunch {
+    int nshots;
+    int nblocks;           // first[nshots]
+    int per_xcd;           // launched blocks / 8
+    int first[kShotsPerLaunch + 1];
+    TileMap maps[kShotsPerLaunch];
+    DevStep steps[kShotsPerLaunch];
+};
+static_assert(sizeof(ShotLaunch) <= 4096, "kernel argument segment");
+
 }  // namespace seis3d

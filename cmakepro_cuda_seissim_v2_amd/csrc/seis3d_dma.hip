@@ -151,15 +151,23 @@ __device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, in
 
 }  // namespace
 
-// Fused velocity half-step (velocityKernel3D + force source epilogue).
+// One workgroup's tile (bx, by) and z-chunk bz of the velocity half-step.
+// LDS of the velocity half-step: the plane ring and the tile's CPML tables.
+template <int BY>
+struct VelShared {
+    VelTiles<BY> ring[3];
+    float cx[6 * 64];          // x CPML table: [half*3 + k][lane]
+    float cyt[6 * BY];         // y CPML table: [half*3 + k][row]
+    float czt[6 * kMaxChunk];  // z CPML table: [half*3 + k][plane - z0]
+};
+
 template <int BY, bool OOP>
-__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma(DevStep P, TileMap M) {
-    __shared__ VelTiles<BY> ring[3];
-    __shared__ float cx[6 * 64];          // x CPML table: [half*3 + k][lane]
-    __shared__ float cyt[6 * BY];         // y CPML table: [half*3 + k][row]
-    __shared__ float czt[6 * kMaxChunk];  // z CPML table: [half*3 + k][plane - z0]
-    int bx, by, bz;
-    if (!tile_of(M, bx, by, bz)) return;
+__device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
+                                              int bz) {
+    VelTiles<BY>* ring = sh.ring;
+    float* cx = sh.cx;
+    float* cyt = sh.cyt;
+    float* czt = sh.czt;
     const DevGrid& g = P.g;
     const int tx = threadIdx.x;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -413,16 +421,22 @@ struct StrOperands {
 
 }  // namespace
 
-// Fused stress half-step (stressKernel3D + explosion source epilogue + the free
-// surface's stress ghost planes).
+// One workgroup's tile (bx, by) and z-chunk bz of the stress half-step.
+template <int BY>
+struct StrShared {
+    StrTiles<BY> ring[3];
+    float cx[6 * 64];
+    float cyt[6 * BY];
+    float czt[6 * kMaxChunk];
+};
+
 template <int BY, bool OOP>
-__global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
-    __shared__ StrTiles<BY> ring[3];
-    __shared__ float cx[6 * 64];
-    __shared__ float cyt[6 * BY];
-    __shared__ float czt[6 * kMaxChunk];
-    int bx, by, bz;
-    if (!tile_of(M, bx, by, bz)) return;
+__device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
+                                            int bz) {
+    StrTiles<BY>* ring = sh.ring;
+    float* cx = sh.cx;
+    float* cyt = sh.cyt;
+    float* czt = sh.czt;
     const DevGrid& g = P.g;
     const int tx = threadIdx.x;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -665,6 +679,77 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     if (bad) atomicOr(P.p.flags, 1u);
 }
 
+// Fused velocity half-step (velocityKernel3D + force source epilogue).
+template <int BY, bool OOP>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma(DevStep P, TileMap M) {
+    __shared__ VelShared<BY> sh;
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    velocity_tile<BY, OOP>(sh, P, M, bx, by, bz);
+}
+
+// Fused stress half-step (stressKernel3D + explosion source epilogue + the free surface's
+// stress ghost planes).
+template <int BY, bool OOP>
+__global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
+    __shared__ StrShared<BY> sh;
+    int bx, by, bz;
+    if (!tile_of(M, bx, by, bz)) return;
+    stress_tile<BY, OOP>(sh, P, M, bx, by, bz);
+}
+
+// The same half-steps for several shots in one launch: the shots' tiles are dealt out as one
+// list (shot s owns logical tiles [L.first[s], L.first[s + 1])), XCD-aware like a single shot.
+// L.steps[s] is shot s's step parameters for this launch (its own grid, fields and source term).
+__device__ __forceinline__ bool shot_tile_of(const ShotLaunch& L, int& s, int& bx, int& by, int& bz) {
+    const int b = int(blockIdx.x);
+    const int logical = (b & 7) * L.per_xcd + (b >> 3);
+    if (logical >= L.nblocks) return false;
+    int k = 0;
+    while (k + 1 < L.nshots && L.first[k + 1] <= logical) ++k;
+    s = __builtin_amdgcn_readfirstlane(k);   // uniform: the shot's parameters load into SGPRs
+    const TileMap& m = L.maps[s];
+    const int t = logical - L.first[s];
+    bx = __builtin_amdgcn_readfirstlane(t % m.nbx);
+    by = __builtin_amdgcn_readfirstlane((t / m.nbx) % m.nby);
+    bz = __builtin_amdgcn_readfirstlane(t / (m.nbx * m.nby));
+    return true;
+}
+
+// Each shot slot has its own copy of the tile body, addressing its parameters at a fixed
+// kernel-argument offset (as a single-shot launch does), so no parameter is indexed at run time.
+template <int BY, int K = 0>
+__device__ __forceinline__ void velocity_shot(VelShared<BY>& sh, const ShotLaunch& L, int s, int bx, int by, int bz) {
+    if constexpr (K < kShotsPerLaunch) {
+        if (s == K) velocity_tile<BY, false>(sh, L.steps[K], L.maps[K], bx, by, bz);
+        else velocity_shot<BY, K + 1>(sh, L, s, bx, by, bz);
+    }
+}
+
+template <int BY, int K = 0>
+__device__ __forceinline__ void stress_shot(StrShared<BY>& sh, const ShotLaunch& L, int s, int bx, int by, int bz) {
+    if constexpr (K < kShotsPerLaunch) {
+        if (s == K) stress_tile<BY, false>(sh, L.steps[K], L.maps[K], bx, by, bz);
+        else stress_shot<BY, K + 1>(sh, L, s, bx, by, bz);
+    }
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma_shots(ShotLaunch L) {
+    __shared__ VelShared<BY> sh;
+    int s, bx, by, bz;
+    if (!shot_tile_of(L, s, bx, by, bz)) return;
+    velocity_shot<BY>(sh, L, s, bx, by, bz);
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void stress_dma_shots(ShotLaunch L) {
+    __shared__ StrShared<BY> sh;
+    int s, bx, by, bz;
+    if (!shot_tile_of(L, s, bx, by, bz)) return;
+    stress_shot<BY>(sh, L, s, bx, by, bz);
+}
+
 hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {
     int dev = 0, v = 0, st = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -683,17 +768,39 @@ hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {
     return e;
 }
 
+TileMap dma_tile_map(const DevGrid& g, int tile_y, int zchunk) {
+    TileMap m;
+    m.nbx = (g.gx - 8 + 63) / 64;
+    m.nby = (g.gy - 8 + tile_y - 1) / tile_y;
+    m.zchunk = zchunk;
+    m.nbz = (g.gz - 8 + zchunk - 1) / zchunk;
+    m.nblocks = m.nbx * m.nby * m.nbz;
+    m.per_xcd = (m.nblocks + 7) / 8;
+    return m;
+}
+
+hipError_t launch_dma_shots(bool velocity, const ShotLaunch& L, int tile_y, hipStream_t s) {
+    if (L.nshots < 1 || L.nshots > kShotsPerLaunch || L.nblocks < 1) return hipErrorInvalidValue;
+    const dim3 grid(unsigned(L.per_xcd) * 8);
+    switch (tile_y) {
+    case 4:
+        if (velocity) hipLaunchKernelGGL(velocity_dma_shots<4>, grid, dim3(64, 4), 0, s, L);
+        else hipLaunchKernelGGL(stress_dma_shots<4>, grid, dim3(64, 4), 0, s, L);
+        break;
+    case 8:
+        if (velocity) hipLaunchKernelGGL(velocity_dma_shots<8>, grid, dim3(64, 8), 0, s, L);
+        else hipLaunchKernelGGL(stress_dma_shots<8>, grid, dim3(64, 8), 0, s, L);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {
     if (zchunk > kMaxChunk) return hipErrorInvalidValue;
-    TileMap m;
-    m.nbx = (P.g.gx - 8 + 63) / 64;
-    m.nby = (P.g.gy - 8 + tile_y - 1) / tile_y;
-    m.zchunk = zchunk;
-    m.nbz = (P.g.gz - 8 + zchunk - 1) / zchunk;
-    m.nblocks = m.nbx * m.nby * m.nbz;
-    const int launched = (m.nblocks + 7) / 8 * 8;
-    m.per_xcd = launched / 8;
-    const dim3 grid(launched);
+    const TileMap m = dma_tile_map(P.g, tile_y, zchunk);
+    const dim3 grid(unsigned(m.per_xcd) * 8);
     const bool oop = out_of_place(P.p);
     switch (tile_y) {
     case 4:

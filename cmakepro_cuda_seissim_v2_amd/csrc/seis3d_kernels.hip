@@ -163,11 +163,8 @@ __global__ void stress_ghost(DevStep P) {
 }
 
 // velocityGhostKernel3D (cuda/Cuda_Seis3D.cu:249-272).
-__global__ void velocity_ghost(DevStep P, float spacing) {
+__device__ __forceinline__ void velocity_ghost_at(const DevStep& P, float spacing, int x, int y) {
     const DevGrid& g = P.g;
-    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
-    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
-    if (x >= g.gx - 4 || y >= g.gy - 4) return;
     float* const* f = P.p.f;
     const long long i = cell(g, x, y, 4);
     const float invh = P.invh;
@@ -185,6 +182,25 @@ __global__ void velocity_ghost(DevStep P, float spacing) {
         f[1][ghost] = f[1][mirror] - float(2 * depth) * spacing * dz_vy;
         f[2][ghost] = f[2][cell(g, x, y, 3 + depth)] - float(2 * depth - 1) * spacing * dz_vz;
     }
+}
+
+__global__ void velocity_ghost(DevStep P, float spacing) {
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    if (x >= P.g.gx - 4 || y >= P.g.gy - 4) return;
+    velocity_ghost_at(P, spacing, x, y);
+}
+
+// The velocity ghosts of several shots in one launch (blockIdx.z = shot; shots without a free
+// surface return at once).
+__global__ void velocity_ghost_shots(const DevStep* __restrict__ steps, const float* __restrict__ spacing) {
+    const int s = int(blockIdx.z);
+    const DevStep& P = steps[s];
+    if (!P.g.free_surface) return;
+    const int x = int(blockIdx.x * blockDim.x + threadIdx.x) + 4;
+    const int y = int(blockIdx.y * blockDim.y + threadIdx.y) + 4;
+    if (x >= P.g.gx - 4 || y >= P.g.gy - 4) return;
+    velocity_ghost_at(P, spacing[s], x, y);
 }
 
 // reflectingGhostKernel3D (cuda/Cuda_Seis3D.cu:152-175), one axis per launch.
@@ -238,13 +254,26 @@ __device__ float sample_scalar(const DevStep& P, long long i, int component) {
 
 // receiverKernel3D (cuda/Cuda_Seis3D.cu:421-428): out[3r..3r+2] = vx, vy, vz.
 // Receivers arrive as padded coordinates.
-__global__ void receivers(DevStep P, const int* __restrict__ rec, int count, float* __restrict__ out) {
-    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (r >= count) return;
+__device__ __forceinline__ void receiver_at(const DevStep& P, const int* __restrict__ rec, int r,
+                                            float* __restrict__ out) {
     const long long i = cell(P.g, rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
     out[3 * r] = sample_scalar(P, i, 1);
     out[3 * r + 1] = sample_scalar(P, i, 2);
     out[3 * r + 2] = sample_scalar(P, i, 0);
+}
+
+__global__ void receivers(DevStep P, const int* __restrict__ rec, int count, float* __restrict__ out) {
+    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= count) return;
+    receiver_at(P, rec, r, out);
+}
+
+// One sample of several shots' receivers in one launch (blockIdx.y = entry).
+__global__ void receivers_shots(const DevStep* __restrict__ steps, const ShotSample* __restrict__ samples) {
+    const ShotSample& e = samples[blockIdx.y];
+    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= e.count) return;
+    receiver_at(steps[e.shot], e.rec, r, e.out);
 }
 
 // generateMaterialsKernel3D (cuda/Cuda_Seis3D.cu:114-147).
@@ -444,6 +473,21 @@ hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s) {
 hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s) {
     const dim3 block(64, 4), grid((P.g.gx - 8 + 63) / 64, (P.g.gy - 8 + 3) / 4);
     hipLaunchKernelGGL(velocity_ghost, grid, block, 0, s, P, spacing);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_ghost_shots(const DevStep* steps, const float* spacing, int nshots, int max_gx, int max_gy,
+                                       hipStream_t s) {
+    if (nshots <= 0) return hipSuccess;
+    const dim3 block(64, 4), grid((max_gx - 8 + 63) / 64, (max_gy - 8 + 3) / 4, nshots);
+    hipLaunchKernelGGL(velocity_ghost_shots, grid, block, 0, s, steps, spacing);
+    return hipGetLastError();
+}
+
+hipError_t launch_receivers_shots(const DevStep* steps, const ShotSample* samples, int entries, int max_count,
+                                  hipStream_t s) {
+    if (entries <= 0 || max_count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(receivers_shots, dim3((max_count + 63) / 64, entries), dim3(64), 0, s, steps, samples);
     return hipGetLastError();
 }
 

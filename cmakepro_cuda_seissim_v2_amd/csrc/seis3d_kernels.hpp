@@ -48,6 +48,10 @@ hipError_t launch_extract_views(const DevStep& P, const ViewParams& vp, long lon
 hipError_t launch_velocity(const DevStep& P, const KernelConfig& k, hipStream_t s);
 // LDS-DMA staged 2.5D family (seis3d_dma.hip), tile_y 4 or 8.
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s);
+// Tile map of the LDS-DMA family for one grid.
+TileMap dma_tile_map(const DevGrid& g, int tile_y, int zchunk);
+// One LDS-DMA half-step (in place) of the L.nshots shots of L, one launch; all share tile_y.
+hipError_t launch_dma_shots(bool velocity, const ShotLaunch& L, int tile_y, hipStream_t s);
 // Resident workgroups of the family-4 pair per CU (the smaller of velocity and
 // stress) and the device's CU count, for sizing launches in whole rounds.
 hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus);
@@ -55,6 +59,19 @@ hipError_t launch_stress(const DevStep& P, const KernelConfig& k, hipStream_t s)
 hipError_t launch_stress_ghost(const DevStep& P, hipStream_t s);
 hipError_t launch_velocity_ghost(const DevStep& P, float spacing, hipStream_t s);
 hipError_t launch_reflecting_walls(const DevStep& P, bool velocities, hipStream_t s);
+// One receiver sample of one shot within a multi-shot launch.
+struct ShotSample {
+    int shot;              // index into the launch's step table
+    int count;             // receivers
+    const int* rec;        // padded coordinates, 3 ints each
+    float* out;            // [receiver][vx, vy, vz]
+};
+// Velocity ghosts of every free-surface shot in `steps` (device array of nshots), one launch.
+hipError_t launch_velocity_ghost_shots(const DevStep* steps, const float* spacing, int nshots, int max_gx, int max_gy,
+                                       hipStream_t s);
+// The receiver samples `samples` (device array of `entries`) of several shots, one launch.
+hipError_t launch_receivers_shots(const DevStep* steps, const ShotSample* samples, int entries, int max_count,
+                                  hipStream_t s);
 hipError_t launch_receivers(const DevStep& P, const int* rec, int count, float* out, hipStream_t s);
 hipError_t launch_generate_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);
 hipError_t launch_extend_materials(const DevStep& P, const MaterialParams& mp, hipStream_t s);

@@ -163,6 +163,7 @@ def library() -> C.CDLL:
             "seis3d_gather_timing": (C.c_int, [P(Ctx), C.c_int, P(C.c_int), P(C.c_double)]),
             "seis3d_ricker": (C.c_float, [C.c_float, C.c_double]),
             "seis3d_create": (C.c_int, [P(Ctx), P(vp)]),
+            "seis3d_batch_create_mode": (C.c_int, [P(vp), P(Source), C.c_int, C.c_int64, C.c_int, P(vp)]),
             "seis3d_bandwidth_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(C.c_double), P(C.c_double)]),
             "seis3d_destroy": (None, [vp]),
             "seis3d_get_ctx": (C.c_int, [vp, P(Ctx)]),
@@ -637,13 +638,14 @@ class Batch:
     created; after launch() + synchronize() every session's gather holds its shot record.
     """
 
-    def __init__(self, sessions, sources, n_steps: int):
+    def __init__(self, sessions, sources, n_steps: int, shared_launches: bool = False):
         self.sessions = list(sessions)
         n = len(self.sessions)
         hs = (C.c_void_p * n)(*[g.handle.value for g in self.sessions])
         srcs = (Source * n)(*sources)
         self._b = C.c_void_p()
-        _check(library().seis3d_batch_create(hs, srcs, n, int(n_steps), C.byref(self._b)))
+        _check(library().seis3d_batch_create_mode(hs, srcs, n, int(n_steps), 1 if shared_launches else 0,
+                                                  C.byref(self._b)))
 
     def launch(self) -> None:
         _check(library().seis3d_batch_launch(self._b))

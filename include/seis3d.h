@@ -288,6 +288,14 @@ typedef struct seis3d_batch seis3d_batch_t;
  * same gather slots. */
 int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources, int count,
                         int64_t n_steps, seis3d_batch_t** out);
+/* Batch modes.  SHOT_STREAMS (what seis3d_batch_create builds): every shot's own launches on its
+ * own stream.  SHARED_LAUNCHES: shots grouped by tile height, up to 4 per group, each group one
+ * launch per half-step for all its shots (the shots' tiles dealt out as one list) on its own
+ * stream; shots with reflecting walls or a learned frame make the batch fall back to
+ * SHOT_STREAMS.  Both give every shot's gather bit for bit as the shot run alone. */
+enum seis3d_batch_mode { SEIS3D_BATCH_SHOT_STREAMS = 0, SEIS3D_BATCH_SHARED_LAUNCHES = 1 };
+int seis3d_batch_create_mode(seis3d_t* const* sessions, const seis3d_source* sources, int count,
+                             int64_t n_steps, int mode, seis3d_batch_t** out);
 int seis3d_batch_launch(seis3d_batch_t* b);
 int seis3d_batch_synchronize(seis3d_batch_t* b);
 void seis3d_batch_destroy(seis3d_batch_t* b);

@@ -35,7 +35,8 @@ def plain_shot(ctx, model, src, rec, steps, stride):
     return np.stack(samples)
 
 
-def test_batched_shots_equal_single_shots(gpu):
+@pytest.mark.parametrize("shared", [False, True], ids=["shot-streams", "shared-launches"])
+def test_batched_shots_equal_single_shots(gpu, shared):
     shots = W.batch_shots(5, seed=11, sizes=[(40, 24, 36), (56, 20, 32), (33, 27, 41), (48, 32, 24), (36, 36, 36)],
                           corr=4.0, receivers=20, pml=8)
     steps, stride = 120, 2
@@ -44,7 +45,7 @@ def test_batched_shots_equal_single_shots(gpu):
         g = prepare(ctx, model, rec)
         g.gather_begin(steps // stride, stride)
         sessions.append(g)
-    batch = S.Batch(sessions, [s[2] for s in shots], steps)
+    batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=shared)
     batch.launch()
     batch.synchronize()
     for (ctx, model, src, rec), g in zip(shots, sessions):

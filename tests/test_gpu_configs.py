@@ -140,7 +140,8 @@ def test_reference_boundary_cases_match_published_ratios(gpu):
 
 
 @pytest.mark.timeout(900)
-def test_config3_sixteen_shot_batch_equals_single_shots(gpu):
+@pytest.mark.parametrize("shared", [False, True], ids=["shot-streams", "shared-launches"])
+def test_config3_sixteen_shot_batch_equals_single_shots(gpu, shared):
     shots = W.batch_shots(16)
     steps, stride = 800, 2
     sessions = []
@@ -152,7 +153,7 @@ def test_config3_sixteen_shot_batch_equals_single_shots(gpu):
         g.gather_begin(steps // stride, stride)
         sessions.append(g)
     log("config 3: 16 shots captured")
-    batch = S.Batch(sessions, [s[2] for s in shots], steps)
+    batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=shared)
     batch.launch()
     batch.synchronize()
     batched = [g.gather_read() for g in sessions]
