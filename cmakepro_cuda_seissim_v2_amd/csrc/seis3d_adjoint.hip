@@ -13,9 +13,12 @@
 // pass applying the transposed stencils (2.5D: z-queues in registers, x/y
 // halos staged in LDS, the next plane's loads in flight during the current
 // one).  Gradients are summed over time per cell in their cheapest form
-// (dJ/dlambda, dJ/dmu of the normal stresses, dJ/dmu_h of each harmonic shear
-// modulus, dJ/dh of each face buoyancy); the harmonic-mean derivatives are
-// applied once at the end.  A free surface adds the transposes of its ghost
+// (dJ/dlambda and dJ/dmu - 2 dJ/dlambda of the normal stresses, dJ/dmu_h of
+// each harmonic shear modulus, dJ/dh of each face buoyancy); the harmonic-mean
+// derivatives are applied once at the end.  The second normal-stress sum is
+// the shear-modulus gradient at fixed P modulus, which is what dJ/dvs needs:
+// each step adds only its cross terms -2 dt sum_{a != b} e_aa L_b, instead of
+// two P-wave sums that would be differenced at the end.  A free surface adds the transposes of its ghost
 // fills (plane-sized passes) and of its constitutive law at z = 4; reflecting
 // walls add the transposes of their mirror fills.
 // Built with -ffp-contract=off, as the forward kernels.
@@ -257,7 +260,8 @@ __global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
         le[2][2] = 0.0f;
         const float glp = dt * (e[0][0] + e[1][1]) * sumL;
         P.g_lam[i] += glp * 4 * m * m / (den * den);
-        P.g_mu[i] += glp * 2 * lam * lam / (den * den) + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
+        // dJ/dmu - 2 dJ/dlambda: 2 lambda^2 - 8 mu^2 = 2 (lambda - 2 mu)(lambda + 2 mu).
+        P.g_dev[i] += glp * 2 * (lam - 2 * m) / den + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
         P.a[5][i] = 0.0f;
     } else {
         const float sumL = Ls[0] + Ls[1] + Ls[2];
@@ -265,7 +269,8 @@ __global__ __launch_bounds__(256) void adjoint_stress_local(AdjStep P) {
         for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
         const float tr = e[0][0] + e[1][1] + e[2][2];
         P.g_lam[i] += dt * tr * sumL;
-        P.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+        // dJ/dmu - 2 dJ/dlambda = 2 dt (sum_a e_aa L_a - tr sumL) = -2 dt sum_{a != b} e_aa L_b.
+        P.g_dev[i] -= dt * 2 * (e[0][0] * (Ls[1] + Ls[2]) + e[1][1] * (Ls[0] + Ls[2]) + e[2][2] * (Ls[0] + Ls[1]));
     }
     le[0][1] = le[1][0] = dt * mxy * Ls[3];
     le[1][2] = le[2][1] = dt * myz * Ls[4];
@@ -447,7 +452,8 @@ __global__ void adjoint_velocity_ghost_fold(AdjStep P) {
     const float ey = aderiv<false>(P.fv[1], i, g.px, P.invh);
     const float gr = -(ex + ey) * l2;
     P.g_lam[i] += gr * 2 * m / (den * den);
-    P.g_mu[i] -= gr * 2 * lam / (den * den);
+    // dr/dmu at fixed lambda + 2 mu is -2 / (lambda + 2 mu).
+    P.g_dev[i] -= gr * 2 / den;
     const long long col = static_cast<long long>(y) * g.px + x;
     P.surf[0][col] = P.invh * l0;
     P.surf[1][col] = P.invh * l1;
@@ -585,7 +591,7 @@ __device__ __forceinline__ bool interior(const DevGrid& g, int x, int y, int z) 
 // axis (d(2ab/(a+b))/da = 2 b^2/(a+b)^2), and dJ/dmu[k] including the four
 // harmonic shear moduli of each plane that use mu[k]
 // (d(4/sum 1/mu_j)/dmu_k = mu_h^2 / (4 mu_k^2)).
-__global__ void adjoint_cells(AdjStep P, float* __restrict__ out_ib, float* __restrict__ out_mu) {
+__global__ void adjoint_cells(AdjStep P, float* __restrict__ out_ib, float* __restrict__ out_dev) {
     const DevGrid& g = P.g;
     const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = int(blockIdx.y);
@@ -611,7 +617,7 @@ __global__ void adjoint_cells(AdjStep P, float* __restrict__ out_ib, float* __re
             gib += P.g_buoy[v][km] * 2 * aa * aa / den;
         }
     }
-    float gmu = interior(g, x, y, z) ? P.g_mu[k] : 0.0f;
+    float gdev = interior(g, x, y, z) ? P.g_dev[k] : 0.0f;
     const int pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};
     const float* r = P.inv_mu;
     const float mk = P.mu[k];
@@ -627,16 +633,19 @@ __global__ void adjoint_cells(AdjStep P, float* __restrict__ out_ib, float* __re
             const long long i = acell(g, q[0], q[1], q[2]);
             const float G = P.g_shear[pp][i];
             const float mh = shear_from_inverse(r[i], r[i + st[A]], r[i + st[B]], r[i + st[A] + st[B]]);
-            if (mh != 0.0f) gmu += G * mh * mh / (4 * mk * mk);
+            if (mh != 0.0f) gdev += G * mh * mh / (4 * mk * mk);
         }
     }
     out_ib[k] = gib;
-    out_mu[k] = gmu;
+    out_dev[k] = gdev;
 }
 
 // Folds the clamp-extended padding onto each physical cell (in padded raster
-// order) and applies the chain rule lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2,
-// 1/rho.
+// order) and applies the chain rule.  The sums are taken in the moduli
+// M = lambda + 2 mu (g_lam = dJ/dM at fixed mu) and mu at fixed M (g_dev), with
+// M = rho vp^2, mu = rho vs^2 and 1/rho, so no difference of two large sums is
+// ever formed: dJ/dvs = 2 rho vs g_dev, dJ/dvp = 2 rho vp g_lam and
+// dJ/drho = vp^2 g_lam + vs^2 g_dev - g_ib / rho^2.
 __global__ void adjoint_fold(FoldParams F) {
     const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = int(blockIdx.y);
@@ -646,14 +655,14 @@ __global__ void adjoint_fold(FoldParams F) {
     const int x0 = x == 0 ? 0 : x + F.ox, x1 = x == F.nx - 1 ? g.gx - 1 : x + F.ox;
     const int y0 = y == 0 ? 0 : y + F.oy, y1 = y == F.ny - 1 ? g.gy - 1 : y + F.oy;
     const int z0 = z == 0 ? 0 : z + F.oz, z1 = z == F.nz - 1 ? g.gz - 1 : z + F.oz;
-    float gib = 0.0f, glam = 0.0f, gmu = 0.0f;
+    float gib = 0.0f, glam = 0.0f, gdev = 0.0f;
     for (int zz = z0; zz <= z1; ++zz)
         for (int yy = y0; yy <= y1; ++yy)
             for (int xx = x0; xx <= x1; ++xx) {
                 const long long i = acell(g, xx, yy, zz);
                 gib += F.g_ib[i];
                 glam += F.g_lam[i];
-                gmu += F.g_mu[i];
+                gdev += F.g_dev[i];
             }
     const long long i = acell(g, x + F.ox, y + F.oy, z + F.oz);
     const float rho = 1.0f / F.inv_rho[i];
@@ -661,8 +670,8 @@ __global__ void adjoint_fold(FoldParams F) {
     const float vp = sqrtf(vp2), vs = sqrtf(vs2);
     const long long j = (static_cast<long long>(z) * F.ny + y) * F.nx + x;
     F.g_vp[j] = glam * 2 * rho * vp;
-    F.g_vs[j] = glam * (-4 * rho * vs) + gmu * 2 * rho * vs;
-    F.g_rho[j] = glam * (vp2 - 2 * vs2) + gmu * vs2 - gib / (rho * rho);
+    F.g_vs[j] = gdev * 2 * rho * vs;
+    F.g_rho[j] = glam * vp2 + gdev * vs2 - gib / (rho * rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -805,9 +814,9 @@ hipError_t launch_residual(const float* gather, const float* observed, float* re
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_cells(const AdjStep& P, float* g_ib, float* out_mu, hipStream_t s) {
+hipError_t launch_adjoint_cells(const AdjStep& P, float* g_ib, float* out_dev, hipStream_t s) {
     hipLaunchKernelGGL(adjoint_cells, dim3(unsigned((P.g.gx + 63) / 64), unsigned(P.g.gy), unsigned(P.g.gz)), dim3(64),
-                       0, s, P, g_ib, out_mu);
+                       0, s, P, g_ib, out_dev);
     return hipGetLastError();
 }
 

@@ -24,8 +24,8 @@ struct AdjStep {
     float* a[9];             // adjoint wavefields
     float* apsi[3];          // adjoint CPML memories (forward psi layout)
     float* t[9];             // filtered adjoint derivatives [3 v + axis]; zero outside the updated cells
-    float* g_lam;            // sum_n dJ/dlambda at each cell
-    float* g_mu;             // sum_n dJ/dmu through the normal stresses
+    float* g_lam;            // sum_n dJ/dlambda at each cell (= dJ/dM at fixed mu, M = lambda + 2 mu)
+    float* g_dev;            // sum_n dJ/dmu - 2 dJ/dlambda through the normal stresses (dJ/dmu at fixed M)
     float* g_shear[3];       // sum_n dJ/dmu_h of the xy, yz, xz harmonic shear moduli
     float* g_buoy[3];        // sum_n dJ/dh of the x, y, z face buoyancies
     float* surf[3];          // free surface: per-column scratch planes of the velocity-ghost transpose
@@ -54,7 +54,7 @@ struct FoldParams {
     const float* mu;
     const float* g_ib;
     const float* g_lam;
-    const float* g_mu;
+    const float* g_dev;
     float* g_vp;             // [nz][ny][nx]
     float* g_vs;
     float* g_rho;
@@ -71,7 +71,7 @@ hipError_t launch_adjoint_inject(const AdjStep& P, const int* rec, int count, co
 // residual = gather - observed, elementwise.
 hipError_t launch_residual(const float* gather, const float* observed, float* residual, long long n, hipStream_t s);
 // Per-cell dJ/d(1/rho) into g_ib and the harmonic shear terms folded into out_mu.
-hipError_t launch_adjoint_cells(const AdjStep& P, float* g_ib, float* out_mu, hipStream_t s);
+hipError_t launch_adjoint_cells(const AdjStep& P, float* g_ib, float* out_dev, hipStream_t s);
 hipError_t launch_adjoint_fold(const FoldParams& F, hipStream_t s);
 hipError_t launch_copy_arrays(const ArrayBatch& b, hipStream_t s);
 hipError_t launch_zero_arrays(const ArrayBatch& b, hipStream_t s);

@@ -108,7 +108,7 @@ struct ShotWork {
     float* a[9]{};
     float* apsi[3]{};
     float* t[9]{};
-    float* acc[8]{};                      // g_lam, g_mu, g_shear[3], g_buoy[3]
+    float* acc[8]{};                      // g_lam, g_dev, g_shear[3], g_buoy[3]
     float* surf[3]{};                     // free-surface scratch planes
 
     void release() noexcept {
@@ -285,7 +285,7 @@ struct Job {
         for (int a = 0; a < 3; ++a) { A.psi[a] = slot.psi[a]; A.coef[a] = base.p.coef[a]; A.apsi[a] = w.apsi[a]; }
         A.inv_rho = base.p.inv_rho; A.lambda = base.p.lambda; A.mu = base.p.mu; A.inv_mu = base.p.inv_mu;
         for (int k = 0; k < 9; ++k) { A.a[k] = w.a[k]; A.t[k] = w.t[k]; }
-        A.g_lam = w.acc[0]; A.g_mu = w.acc[1];
+        A.g_lam = w.acc[0]; A.g_dev = w.acc[1];
         for (int k = 0; k < 3; ++k) { A.g_shear[k] = w.acc[2 + k]; A.g_buoy[k] = w.acc[5 + k]; }
         for (int k = 0; k < 3; ++k) A.surf[k] = w.surf[k];
         A.dt = base.dt;
@@ -509,14 +509,14 @@ int seis3d_fwi_gradient(seis3d_t* t, const seis3d_shot* shots, int count, int64_
             // Per-cell 1/rho and shear terms into two temporaries, then fold + chain rule.
             ShotWork& w0 = work[0];
             seis3d::AdjStep A = job.adjoint_operands(w0, w0.slots[0], 0);
-            A.g_lam = total[0]; A.g_mu = total[1];
+            A.g_lam = total[0]; A.g_dev = total[1];
             for (int k = 0; k < 3; ++k) { A.g_shear[k] = total[2 + k]; A.g_buoy[k] = total[5 + k]; }
             check(seis3d::launch_adjoint_cells(A, w0.t[0], w0.t[1], w0.stream), "kernel launch");
             seis3d::FoldParams F{};
             F.g = base.g;
             F.nx = c.nx; F.ny = c.ny; F.nz = c.nz; F.ox = c.ox; F.oy = c.oy; F.oz = c.oz;
             F.inv_rho = base.p.inv_rho; F.lambda = base.p.lambda; F.mu = base.p.mu;
-            F.g_ib = w0.t[0]; F.g_lam = total[0]; F.g_mu = w0.t[1];
+            F.g_ib = w0.t[0]; F.g_lam = total[0]; F.g_dev = w0.t[1];
             const std::size_t phys = std::size_t(c.nx) * c.ny * c.nz;
             F.g_vp = w0.t[2]; F.g_vs = w0.t[3]; F.g_rho = w0.t[4];
             if (phys > L.cells) invalid("Seis3D: internal gradient layout error.");

@@ -537,13 +537,14 @@ public:
     struct Adjoint {
         std::vector<Real> f[9];      // adjoints of the nine wavefields
         std::vector<Real> cpml[3];   // adjoints of the CPML memories (same slab layout)
-        std::vector<Real> g_ib, g_lam, g_mu;
+        std::vector<Real> g_ib, g_lam;    // dJ/d(1/rho), dJ/dlambda (= dJ/dM at fixed mu, M = lambda + 2 mu)
+        std::vector<Real> g_dev;          // dJ/dmu at fixed M = dJ/dmu - 2 dJ/dlambda
     };
     Adjoint make_adjoint() const {
         Adjoint A;
         for (auto& v : A.f) v.assign(c.cells, Real(0));
         for (int a = 0; a < 3; ++a) A.cpml[a].assign(cpml[a].size(), Real(0));
-        A.g_ib.assign(c.cells, Real(0)); A.g_lam.assign(c.cells, Real(0)); A.g_mu.assign(c.cells, Real(0));
+        A.g_ib.assign(c.cells, Real(0)); A.g_lam.assign(c.cells, Real(0)); A.g_dev.assign(c.cells, Real(0));
         return A;
     }
 
@@ -724,7 +725,7 @@ public:
             const Real ey = derivative<false>(v.f[1].data(), i, gx, invH);
             const Real gr = -(ex + ey) * l_dzvz;
             A.g_lam[i] += gr * 2 * m / (den * den);
-            A.g_mu[i] -= gr * 2 * lam / (den * den);
+            A.g_dev[i] -= gr * 2 / den;                     // dr/dmu at fixed lambda + 2 mu
         }
     }
 
@@ -760,14 +761,14 @@ public:
                 le[1][1] = dt * (lp * sumL + 2 * m * Ls[1]);
                 const Real glp = dt * exy * sumL;
                 A.g_lam[i] += glp * 4 * m * m / (den * den);
-                A.g_mu[i] += glp * 2 * lam * lam / (den * den) + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
+                A.g_dev[i] += glp * 2 * (lam - 2 * m) / den + dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1]);
                 A.f[5][i] = 0;
             } else {
                 const Real sumL = Ls[0] + Ls[1] + Ls[2];
                 for (int v = 0; v < 3; ++v) le[v][v] = dt * (lam * sumL + 2 * m * Ls[v]);
                 const Real tr = e[0][0] + e[1][1] + e[2][2];
                 A.g_lam[i] += dt * tr * sumL;
-                A.g_mu[i] += dt * 2 * (e[0][0] * Ls[0] + e[1][1] * Ls[1] + e[2][2] * Ls[2]);
+                A.g_dev[i] -= dt * 2 * (e[0][0] * (Ls[1] + Ls[2]) + e[1][1] * (Ls[0] + Ls[2]) + e[2][2] * (Ls[0] + Ls[1]));
             }
             const Real mxy = shear_mu(i, st[0], st[1]), myz = shear_mu(i, st[1], st[2]), mxz = shear_mu(i, st[0], st[2]);
             le[0][1] = le[1][0] = dt * mxy * Ls[3];
@@ -780,7 +781,7 @@ public:
             for (int p = 0; p < 3; ++p) {
                 if (mh[p] == 0) continue;
                 const std::size_t cells4[4] = {i, i + pairs[p][0], i + pairs[p][1], i + pairs[p][0] + pairs[p][1]};
-                for (auto k : cells4) A.g_mu[k] += gh[p] * mh[p] * mh[p] / (4 * mu[k] * mu[k]);
+                for (auto k : cells4) A.g_dev[k] += gh[p] * mh[p] * mh[p] / (4 * mu[k] * mu[k]);
             }
             for (int v = 0; v < 3; ++v) for (int a = 0; a < 3; ++a)
                 ld[3 * v + a][i] = adjoint_filter(A, le[v][a], x, y, z, a, 3 + v, a != v);
@@ -890,14 +891,14 @@ public:
         }
         load(states.back());
         // Fold the padding (materials are clamp-extended copies) onto physical cells.
-        std::vector<Real> gib(std::size_t(c.nx) * c.ny * c.nz, 0), glam(gib.size(), 0), gmu(gib.size(), 0);
+        std::vector<Real> gib(std::size_t(c.nx) * c.ny * c.nz, 0), glam(gib.size(), 0), gdev(gib.size(), 0);
         for (std::size_t i = 0; i < c.cells; ++i) {
             const int x = int(i % c.gx), y = int((i / c.gx) % c.gy), z = int(i / (std::size_t(c.gx) * c.gy));
             const int px = std::max(0, std::min(c.nx - 1, x - c.ox));
             const int py = std::max(0, std::min(c.ny - 1, y - c.oy));
             const int pz = std::max(0, std::min(c.nz - 1, z - c.oz));
             const std::size_t j = (std::size_t(pz) * c.ny + py) * c.nx + px;
-            gib[j] += A.g_ib[i]; glam[j] += A.g_lam[i]; gmu[j] += A.g_mu[i];
+            gib[j] += A.g_ib[i]; glam[j] += A.g_lam[i]; gdev[j] += A.g_dev[i];
         }
         for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
             const std::size_t j = (std::size_t(z) * c.ny + y) * c.nx + x;
@@ -906,8 +907,9 @@ public:
             const Real vs2 = mu[i] / rho, vp2 = (lambda[i] + 2 * mu[i]) / rho;
             const Real vp = std::sqrt(vp2), vs = std::sqrt(vs2);
             g_vp[j] = glam[j] * 2 * rho * vp;
-            g_vs[j] = glam[j] * (-4 * rho * vs) + gmu[j] * 2 * rho * vs;
-            g_rho[j] = glam[j] * (vp2 - 2 * vs2) + gmu[j] * vs2 - gib[j] / (rho * rho);
+            // M = lambda + 2 mu = rho vp^2 (g_lam = dJ/dM at fixed mu), mu = rho vs^2 at fixed M (g_dev).
+            g_vs[j] = gdev[j] * 2 * rho * vs;
+            g_rho[j] = glam[j] * vp2 + gdev[j] * vs2 - gib[j] / (rho * rho);
         }
         return misfit;
     }
