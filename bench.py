@@ -46,8 +46,8 @@ def parse():
                    help="step: the fused 3D step (default line); batch: 16 uneven shots, graph vs plain loop")
     p.add_argument("--shots", type=int, default=0, help="shots of the batch (16) or fwi (8) workload")
     p.add_argument("--checkpoint", type=int, default=50, help="FWI checkpoint interval (steps)")
-    p.add_argument("--fd", type=int, default=0,
-                   help="FWI: central-difference checks along the gradient (vp, vs, rho, all three)")
+    p.add_argument("--fd", action="store_true",
+                   help="FWI: central-difference checks along the vp, vs and rho parts of the gradient")
     p.add_argument("--no-copy-ceiling", action="store_true",
                    help="skip the stream-kernel ceiling measurement (profiled runs)")
     p.add_argument("--copy-ceiling", action="store_true",
@@ -205,6 +205,32 @@ def fwi_misfit(S, ctx, model, shots, observed, steps, stride):
     return total
 
 
+def fwi_fd_checks(S, W, ctx, start, gradient, shots, observed, steps, stride, params=(0, 1)) -> list:
+    """Central differences of the misfit along `fwi_check_direction` of each parameter in `params`,
+    at relative steps 5e-4 and 1e-3, Richardson-extrapolated (fourth order in the step)."""
+    import numpy as np
+    base = [np.asarray(a, np.float32) for a in start]
+    out = []
+    for k in params:
+        d = W.fwi_check_direction(ctx, base, gradient[1 + k], k)
+        ratios = []
+        for eps in (5e-4, 1e-3):
+            js, moved = [], []
+            for sign in (1, -1):
+                m = list(base)
+                m[k] = (base[k].astype(np.float64) + sign * eps * d).astype(np.float32)
+                moved.append(m[k])
+                js.append(fwi_misfit(S, ctx, tuple(m), shots, observed, steps, stride))
+            pred = float(np.sum(gradient[1 + k].astype(np.float64) * (moved[0].astype(np.float64) - moved[1])))
+            ratios.append((js[0] - js[1]) / pred)
+        extrap = (4 * ratios[0] - ratios[1]) / 3
+        out.append({"direction": ["vp", "vs", "rho"][k] + " part of the gradient, below 20-30 cells",
+                    "fd_over_adjoint": [round(r, 7) for r in ratios], "extrapolated": round(extrap, 7),
+                    "rel_error": float(f"{abs(extrap - 1):.3e}")})
+        print(f"bench: fd direction {out[-1]}", file=sys.stderr, flush=True)
+    return out
+
+
 def fwi_bench(args) -> None:
     """FWI gradient of 8 shots on 200x100x200, free top (1200 steps, checkpoint every 50 steps).
 
@@ -247,38 +273,7 @@ def fwi_bench(args) -> None:
     summed = [np.sum([s[k].astype(np.float64) for s in singles], axis=0) for k in (1, 2, 3)]
     agree = max(float(np.linalg.norm(batched[k + 1] - summed[k]) / np.linalg.norm(summed[k])) for k in range(3))
     n_upd = (ctx.gx - 8) * (ctx.gy - 8) * (ctx.gz - 8)
-    fd = []
-    if args.fd:
-        base = [np.asarray(a, np.float32) for a in start]
-        # Directions vanish where vp approaches the context's vp_max (the deepest rows of the
-        # starting model sit on it): uploads refuse a faster vp, and vp_max fixes dt.
-        taper = np.clip((ctx.max_vp - base[0].astype(np.float64)) / (0.05 * ctx.max_vp), 0.0, 1.0)
-        # Directions along the adjoint gradient itself (per parameter, scaled to a largest relative
-        # change of 1): over 1200 steps the misfit is strongly nonlinear (many wavelengths of
-        # travel), so a random direction's small projection on the gradient drowns in the cubic
-        # term; along the gradient the first-order term dominates.
-        for j in range(args.fd):
-            mask = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)][j % 4]
-            d = []
-            for k in range(3):
-                gk = batched[1 + k].astype(np.float64) * taper
-                peak = float(np.max(np.abs(gk))) if mask[k] else 0.0
-                d.append(gk / peak * base[k] * mask[k] if peak > 0 else np.zeros_like(gk))
-            ratios = []
-            for eps in (5e-4, 1e-3):
-                js, moved = [], []
-                for sign in (1, -1):
-                    m = [(b.astype(np.float64) + sign * eps * dk).astype(np.float32) for b, dk in zip(base, d)]
-                    moved.append(m)
-                    js.append(fwi_misfit(S, ctx, tuple(m), shots, observed, steps, stride))
-                pred = sum(float(np.sum(gk.astype(np.float64) * (p.astype(np.float64) - q.astype(np.float64))))
-                           for gk, p, q in zip(batched[1:], moved[0], moved[1]))
-                ratios.append((js[0] - js[1]) / pred)
-            extrap = (4 * ratios[0] - ratios[1]) / 3
-            fd.append({"direction": "gradient, " + ["vp", "vs", "rho", "vp+vs+rho"][j % 4],
-                       "fd_over_adjoint": [round(r, 7) for r in ratios], "extrapolated": round(extrap, 7),
-                       "rel_error": float(f"{abs(extrap - 1):.3e}")})
-            print(f"bench: fd direction {fd[-1]}", file=sys.stderr, flush=True)
+    fd = fwi_fd_checks(S, W, ctx, start, batched, shots, observed, steps, stride, params=(0, 1, 2)) if args.fd else []
     out = {
         "metric": "FWI gradient shots/s (vp, vs, rho; discrete adjoint + checkpoints), batched vs plain per-shot loop",
         "value": round(len(shots) / t_batch, 4),
@@ -791,6 +786,8 @@ def fwi_leg(S, W) -> dict:
         "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
         "misfit": batched[0],
         "batched_vs_summed_singles_rel_l2": float(f"{agree:.3e}"),
+        "fd_checks": fwi_fd_checks(S, W, ctx, start, batched, shots, observed, steps, stride),
+        "fd_tolerance": 1e-3,
     }
     print(f"bench: fwi {out}", file=sys.stderr, flush=True)
     return out

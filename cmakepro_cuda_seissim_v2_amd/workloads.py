@@ -142,3 +142,22 @@ def fwi_workload(count: int = 8, nx: int = 200, ny: int = 100, nz: int = 200, se
         shots.append((S.default_source(x=min(max(sx, 1), nx - 2), y=ny // 2, z=5, kind=S.EXPLOSION,
                                        amplitude=1.0e8), rec))
     return ctx, start, true, shots
+
+
+def fwi_check_direction(ctx: S.Ctx, model, gradient, param: int, top: int = 20, ramp: int = 10) -> np.ndarray:
+    """Central-difference direction for a full-size FWI gradient check along parameter `param`
+    (0 vp, 1 vs, 2 rho): the gradient itself, scaled so that the largest relative change of that
+    parameter is 1, zero where vp reaches the context's vp_max (uploads refuse a faster vp, and
+    vp_max fixes dt), and ramped from zero at depth `top` to one at `top + ramp`.
+
+    The ramp keeps the direction away from the explosions (depth 5) and the surface receivers:
+    there the misfit is so strongly nonlinear in the model that central differences at any step
+    FP32 misfits can resolve are dominated by their third-order term.  Below it they converge
+    quadratically.
+    """
+    base = np.asarray(model[param], np.float64)
+    vp = np.asarray(model[0], np.float64)
+    taper = np.clip((ctx.max_vp - vp) / (0.05 * ctx.max_vp), 0.0, 1.0)
+    depth = np.arange(base.shape[0], dtype=np.float64)[:, None, None]
+    g = np.asarray(gradient, np.float64) * taper * np.clip((depth - top) / ramp, 0.0, 1.0)
+    return g / float(np.max(np.abs(g))) * base

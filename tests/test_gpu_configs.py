@@ -11,7 +11,7 @@
   same shot run on its own, bit for bit.
 * Config 4: the FWI gradient, 8 shots on 200x100x200 (free top), 1200 steps, checkpoints every 50:
   the batched request equals the sum of single-shot requests, and central differences of the
-  misfit along the gradient agree with the adjoint.
+  misfit along the gradient's vp and vs parts agree with the adjoint to 1e-3.
 * Config 5: the learned frame on the 300x300x400 grid, 600 steps: stable, and it reflects less
   than the same frame without its network.
 """
@@ -209,23 +209,25 @@ def test_config4_fwi_full_size(gpu):
         e = rel_l2(batched[k], np.sum([s[k].astype(np.float64) for s in singles], axis=0))
         log(f"config 4: batched vs summed singles, parameter {k}: rel-L2 {e:.2e}")
         assert e < 1e-6
-    # Central differences along the gradient's vp part, Richardson-extrapolated over two steps.
+    # Central differences along the gradient's vp and vs parts (below the sources and receivers,
+    # where the misfit is smooth enough for steps that FP32 misfits resolve), Richardson-
+    # extrapolated over two steps.
     base = [np.asarray(a, np.float32) for a in start]
-    taper = np.clip((ctx.max_vp - base[0].astype(np.float64)) / (0.05 * ctx.max_vp), 0.0, 1.0)
-    gk = batched[1].astype(np.float64) * taper
-    d = gk / float(np.max(np.abs(gk))) * base[0]
-    ratios = []
-    for eps in (5e-4, 1e-3):
-        js, moved = [], []
-        for sign in (1, -1):
-            m = (base[0].astype(np.float64) + sign * eps * d).astype(np.float32)
-            moved.append(m)
-            js.append(fwi_misfit(ctx, (m, base[1], base[2]), shots, observed, steps, stride))
-        pred = float(np.sum(batched[1].astype(np.float64) * (moved[0].astype(np.float64) - moved[1])))
-        ratios.append((js[0] - js[1]) / pred)
-    extrap = (4 * ratios[0] - ratios[1]) / 3
-    log(f"config 4: FD/adjoint along the vp gradient {ratios}, extrapolated {extrap:.6f}")
-    assert abs(extrap - 1) < 1e-3
+    for k, name in ((0, "vp"), (1, "vs")):
+        d = W.fwi_check_direction(ctx, base, batched[1 + k], k)
+        ratios = []
+        for eps in (5e-4, 1e-3):
+            js, moved = [], []
+            for sign in (1, -1):
+                m = list(base)
+                m[k] = (base[k].astype(np.float64) + sign * eps * d).astype(np.float32)
+                moved.append(m[k])
+                js.append(fwi_misfit(ctx, tuple(m), shots, observed, steps, stride))
+            pred = float(np.sum(batched[1 + k].astype(np.float64) * (moved[0].astype(np.float64) - moved[1])))
+            ratios.append((js[0] - js[1]) / pred)
+        extrap = (4 * ratios[0] - ratios[1]) / 3
+        log(f"config 4: FD/adjoint along the {name} gradient {ratios}, extrapolated {extrap:.7f}")
+        assert abs(extrap - 1) < 1e-3, (name, ratios, extrap)
 
 
 @pytest.mark.timeout(900)
