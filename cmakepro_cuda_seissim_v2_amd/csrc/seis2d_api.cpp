@@ -183,20 +183,24 @@ seis2d::Src2D single_source(const seis2d_t* t, const seis2d_ctx& c, int it, cons
     return s;
 }
 
-void enqueue_step(seis2d_t* t, const seis2d::Dev2D& g, const seis2d::Src2D& s, int active_sources, const seis2d_ctx& c) {
+// One step: the stress half-step (with the type-3 mirror), the velocity half-step, then one
+// workgroup for the sources, the zero frame and, when `rec` is given, the receiver samples.
+void enqueue_step(seis2d_t* t, const seis2d::Dev2D& g, const seis2d::Src2D& s, int active_sources, const seis2d_ctx& c,
+                  const seis2d::Rec2D* rec = nullptr) {
     check(seis2d::launch_stress2d(g, t->stream), "kernel launch");
-    check(seis2d::launch_velocity2d(g, t->stream), "kernel launch");
-    seis2d::Src2D single = s;
+    seis2d::Tail2D tail{};
+    tail.single = s;
     if (active_sources >= 0) {
-        single.active = 0;
+        tail.single.active = 0;
         if (active_sources > 0) {
-            float sa, ca;
-            angle_terms(c.src_angle, sa, ca);
-            check(seis2d::launch_multi_sources2d(g, t->dev_sources, active_sources, c.src_type, sa, ca, t->stream),
-                  "kernel launch");
+            tail.list = t->dev_sources;
+            tail.count = active_sources;
+            tail.kind = c.src_type;
+            angle_terms(c.src_angle, tail.sin_a, tail.cos_a);
         }
     }
-    check(seis2d::launch_finish2d(g, single, t->stream), "kernel launch");
+    if (rec) tail.rec = *rec;
+    check(seis2d::launch_velocity2d(g, tail, t->stream), "kernel launch");
 }
 
 void record_step(seis2d_t* t, int num_rcv) {
@@ -486,10 +490,7 @@ int seis2d_run(seis2d_t* t, const seis2d_ctx* ctx, const float* wavelet, int wav
             s.active = 1;
             s.table = dev_table;
             const seis2d::Rec2D rec{t->rcv, nrec, tvx, tvz, n_steps, first_it};
-            auto one_step = [&] {
-                enqueue_step(t, g, s, -1, c);
-                check(seis2d::launch_receivers2d(g, rec, t->stream), "kernel launch");
-            };
+            auto one_step = [&] { enqueue_step(t, g, s, -1, c, &rec); };
             const int chunk = std::min(kChunk, n_steps);
             check(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal), "begin capture");
             try {

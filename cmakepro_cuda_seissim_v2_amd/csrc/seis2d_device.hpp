@@ -63,11 +63,22 @@ struct Rec2D {
     int first;            // step index of trace column 0
 };
 
+// The plane-sized work that follows a velocity half-step (applyPointSource or
+// injectMultipleSourcesKernel, zeroBoundaryKernel, extractReceiverSamplesKernel),
+// run by one workgroup.
+struct Tail2D {
+    Src2D single;            // the single source (active = 0: none)
+    const MultiSrc* list;    // multi-source mode: list[count] (count = 0: off)
+    int count;
+    int kind;
+    float sin_a, cos_a;
+    Rec2D rec;               // rec.count = 0: no receiver samples this step
+};
+
+// Stress half-step (type 3 with the surface on: then the ghost-row mirror).
 hipError_t launch_stress2d(const Dev2D& g, hipStream_t s);
-hipError_t launch_velocity2d(const Dev2D& g, hipStream_t s);
-hipError_t launch_finish2d(const Dev2D& g, const Src2D& src, hipStream_t s);
-hipError_t launch_multi_sources2d(const Dev2D& g, const MultiSrc* list, int count, int kind, float sin_a, float cos_a,
-                                  hipStream_t s);
+// Velocity half-step, then the tail.
+hipError_t launch_velocity2d(const Dev2D& g, const Tail2D& tail, hipStream_t s);
 hipError_t launch_receivers2d(const Dev2D& g, const Rec2D& r, hipStream_t s);
 
 }  // namespace seis2d

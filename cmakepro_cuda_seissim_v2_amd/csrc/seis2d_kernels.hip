@@ -8,6 +8,10 @@
 // -ffp-contract=off and IEEE division, the results equal the FP32 oracle's.
 // The reference build uses fast-math (approximate division), which this
 // build does not reproduce.
+//
+// A step is three launches (four with the type-3 ghost mirror): the stress
+// and velocity half-steps, then one workgroup that injects the sources,
+// zeroes the frame and takes the receiver samples, in that order.
 #include "seis2d_device.hpp"
 
 namespace seis2d {
@@ -120,9 +124,7 @@ __global__ __launch_bounds__(256) void stress_type2(Dev2D g) {
 }
 
 // updateType2VelocityKernel (:364-412).
-__global__ __launch_bounds__(256) void velocity_type2(Dev2D g) {
-    int i, j;
-    if (!cell(g, i, j, kR)) return;
+__device__ __forceinline__ void hybrid_velocity(const Dev2D& g, int i, int j) {
     const int k = i * g.nx + j;
     float* const* f = g.f;
     const float sxz_x = fwd1(g, f[14], k, 1), szz_z = fwd1(g, f[13], k, g.nx);
@@ -142,11 +144,13 @@ __global__ __launch_bounds__(256) void velocity_type2(Dev2D g) {
     }
 }
 
-// updateType3StressKernel (:416-487), free surface on.
-__global__ __launch_bounds__(256) void stress_type3(Dev2D g) {
-    count_step(g);
+__global__ __launch_bounds__(256) void velocity_type2(Dev2D g) {
     int i, j;
-    if (!cell(g, i, j, g.fs)) return;
+    if (cell(g, i, j, kR)) hybrid_velocity(g, i, j);
+}
+
+// updateType3StressKernel (:416-487), free surface on.
+__device__ __forceinline__ void surface_stress(const Dev2D& g, int i, int j) {
     const int k = i * g.nx + j;
     const bool near = i <= g.fs + 5 && g.up;
     const float dzv = near ? 0.0f : g.dz[i], dzhv = near ? 0.0f : g.dzh[i];
@@ -167,10 +171,8 @@ __global__ __launch_bounds__(256) void stress_type3(Dev2D g) {
     f[9][k] = pml(f[9][k], g.inv_dt, dzhv, m * dvx_dz);
 }
 
-// mirrorType3StressKernel (:491-508).
-__global__ void mirror_type3(Dev2D g) {
-    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j < kR || j >= g.nx - kR) return;
+// mirrorType3StressKernel (:491-508) for column j.
+__device__ __forceinline__ void mirror_column(const Dev2D& g, int j) {
     float* const* f = g.f;
     for (int l = 1; l <= 4; ++l) {
         const int above = (g.fs - l) * g.nx + j, below = (g.fs + l) * g.nx + j;
@@ -184,10 +186,19 @@ __global__ void mirror_type3(Dev2D g) {
     }
 }
 
-// updateType3VelocityKernel (:512-563): air rows above the surface are zeroed.
-__global__ __launch_bounds__(256) void velocity_type3(Dev2D g) {
+__global__ __launch_bounds__(256) void stress_type3(Dev2D g) {
+    count_step(g);
     int i, j;
-    if (!cell(g, i, j, kR)) return;
+    if (cell(g, i, j, g.fs)) surface_stress(g, i, j);
+}
+
+__global__ void mirror_type3(Dev2D g) {
+    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= kR && j < g.nx - kR) mirror_column(g, j);
+}
+
+// updateType3VelocityKernel (:512-563): air rows above the surface are zeroed.
+__device__ __forceinline__ void surface_velocity(const Dev2D& g, int i, int j) {
     const int k = i * g.nx + j;
     if (i < g.fs) {
         g.f[0][k] = 0.0f; g.f[1][k] = 0.0f; g.f[2][k] = 0.0f; g.f[3][k] = 0.0f;
@@ -195,6 +206,11 @@ __global__ __launch_bounds__(256) void velocity_type3(Dev2D g) {
     }
     const bool near = i <= g.fs + 5 && g.up;
     split_velocity(g, j, k, near ? 0.0f : g.dz[i], near ? 0.0f : g.dzh[i]);
+}
+
+__global__ __launch_bounds__(256) void velocity_type3(Dev2D g) {
+    int i, j;
+    if (cell(g, i, j, kR)) surface_velocity(g, i, j);
 }
 
 // applyPointSource (:567-636); `atomic` for overlapping multi-source injection.
@@ -237,35 +253,9 @@ __device__ void point_source(const Dev2D& g, float value, int idx, int kind, flo
     }
 }
 
-// The single source (injectSourceKernel, :638-640) fused with zeroBoundaryKernel
-// (:643-667): the source cell is interior, the zeroed frame is not, so the
-// two touch disjoint cells.
-__global__ void finish(Dev2D g, Src2D s) {
-    const int idx = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (idx == 0 && s.active) {
-        const float v = s.table ? s.table[*g.counter] : s.value;
-        point_source<false>(g, v, s.idx, s.kind, s.sin_a, s.cos_a);
-    }
-    const bool split = g.type == 1 || g.type == 3;
-    auto zero = [&](int k) {
-        if (split) { g.f[0][k] = 0.0f; g.f[1][k] = 0.0f; g.f[2][k] = 0.0f; g.f[3][k] = 0.0f; }
-        else { g.f[10][k] = 0.0f; g.f[11][k] = 0.0f; }
-    };
-    if (idx < g.nx) { zero(idx); zero((g.nz - 1) * g.nx + idx); }
-    if (idx < g.nz) { zero(idx * g.nx); zero(idx * g.nx + (g.nx - 1)); }
-}
-
-// injectMultipleSourcesKernel (:822-834) with host-evaluated Ricker values.
-__global__ void multi_sources(Dev2D g, const MultiSrc* __restrict__ list, int count, int kind, float sin_a, float cos_a) {
-    const int r = int(threadIdx.x);
-    if (r >= count) return;
-    point_source<true>(g, list[r].value, list[r].idx, kind, sin_a, cos_a);
-}
-
-// extractReceiverSamplesKernel (:793-818); trace mode writes column *counter - first.
-__global__ void receivers(Dev2D g, Rec2D R) {
-    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (r >= R.count) return;
+// extractReceiverSamplesKernel (:793-818) for receiver r; trace mode writes
+// column *counter - first.
+__device__ __forceinline__ void sample_receiver(const Dev2D& g, const Rec2D& R, int r) {
     const int k = R.idx[r];
     const long long o = R.len ? static_cast<long long>(r) * R.len + (*g.counter - R.first) : r;
     if (k < 0 || k >= g.n) {
@@ -276,6 +266,39 @@ __global__ void receivers(Dev2D g, Rec2D R) {
     const bool split = g.type == 1 || g.type == 3;
     R.vx[o] = split ? g.f[0][k] + g.f[1][k] : g.f[10][k];
     R.vz[o] = split ? g.f[2][k] + g.f[3][k] : g.f[11][k];
+}
+
+// One workgroup after the velocity half-step, in the reference's order: the
+// source(s) (injectSourceKernel :638-640 or injectMultipleSourcesKernel
+// :822-834 with host-evaluated Ricker values), zeroBoundaryKernel (:643-667),
+// then the receiver samples.  Sources are interior cells and the zeroed frame
+// is not, so the two touch disjoint cells; the receivers read after both.
+constexpr int kTailThreads = 256;
+
+__global__ __launch_bounds__(kTailThreads) void step_tail(Dev2D g, Tail2D T) {
+    const int tid = int(threadIdx.x), nt = kTailThreads;
+    if (T.count > 0) {
+        for (int r = tid; r < T.count; r += nt)
+            point_source<true>(g, T.list[r].value, T.list[r].idx, T.kind, T.sin_a, T.cos_a);
+    } else if (tid == 0 && T.single.active) {
+        const float v = T.single.table ? T.single.table[*g.counter] : T.single.value;
+        point_source<false>(g, v, T.single.idx, T.single.kind, T.single.sin_a, T.single.cos_a);
+    }
+    const bool split = g.type == 1 || g.type == 3;
+    auto zero = [&](int k) {
+        if (split) { g.f[0][k] = 0.0f; g.f[1][k] = 0.0f; g.f[2][k] = 0.0f; g.f[3][k] = 0.0f; }
+        else { g.f[10][k] = 0.0f; g.f[11][k] = 0.0f; }
+    };
+    for (int c = tid; c < g.nx; c += nt) { zero(c); zero((g.nz - 1) * g.nx + c); }
+    for (int r = tid; r < g.nz; r += nt) { zero(r * g.nx); zero(r * g.nx + (g.nx - 1)); }
+    __syncthreads();
+    for (int r = tid; r < T.rec.count; r += nt) sample_receiver(g, T.rec, r);
+}
+
+// Receiver samples on their own (the per-step extractReceiverSamples call).
+__global__ void receivers(Dev2D g, Rec2D R) {
+    const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < R.count) sample_receiver(g, R, r);
 }
 
 dim3 grid_of(const Dev2D& g) { return dim3(unsigned((g.nx + 63) / 64), unsigned((g.nz + 3) / 4)); }
@@ -292,23 +315,11 @@ hipError_t launch_stress2d(const Dev2D& g, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_velocity2d(const Dev2D& g, hipStream_t s) {
+hipError_t launch_velocity2d(const Dev2D& g, const Tail2D& tail, hipStream_t s) {
     if (g.type == 2) hipLaunchKernelGGL(velocity_type2, grid_of(g), dim3(64, 4), 0, s, g);
     else if (g.type == 3 && g.up) hipLaunchKernelGGL(velocity_type3, grid_of(g), dim3(64, 4), 0, s, g);
     else hipLaunchKernelGGL(velocity_type1, grid_of(g), dim3(64, 4), 0, s, g);
-    return hipGetLastError();
-}
-
-hipError_t launch_finish2d(const Dev2D& g, const Src2D& src, hipStream_t s) {
-    const int m = g.nx > g.nz ? g.nx : g.nz;
-    hipLaunchKernelGGL(finish, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s, g, src);
-    return hipGetLastError();
-}
-
-hipError_t launch_multi_sources2d(const Dev2D& g, const MultiSrc* list, int count, int kind, float sin_a, float cos_a,
-                                  hipStream_t s) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(multi_sources, dim3(1), dim3(256), 0, s, g, list, count, kind, sin_a, cos_a);
+    hipLaunchKernelGGL(step_tail, dim3(1), dim3(kTailThreads), 0, s, g, tail);
     return hipGetLastError();
 }
 
