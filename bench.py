@@ -144,12 +144,12 @@ def batch_bench(args) -> None:
     # Batched graphs: every shot on its own stream (the default), and shared launches (groups of
     # shots stepped by one launch per half-step).
     times, same = {}, True
-    for mode in ("shot_streams", "shared_launches"):
+    for mode in ("shot_streams", "shared_launches", "work_queue"):
         for g in sessions:
             g.reset()
             g.gather_end()
             g.gather_begin(steps // stride, stride)
-        batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=mode == "shared_launches")
+        batch = S.Batch(sessions, [s[2] for s in shots], steps, mode=mode)
         for g in sessions:
             g.synchronize()
         t0 = time.perf_counter()
@@ -180,6 +180,8 @@ def batch_bench(args) -> None:
             "speedup_vs_plain": round(t_plain / t_batch, 3),
             "shared_launches_shots_per_s": round(len(shots) / times["shared_launches"], 4),
             "shared_launches_speedup_vs_plain": round(t_plain / times["shared_launches"], 3),
+            "work_queue_shots_per_s": round(len(shots) / times["work_queue"], 4),
+            "work_queue_speedup_vs_plain": round(t_plain / times["work_queue"], 3),
             "gathers_bit_identical": bool(same),
             "gcell_updates_per_s_batched": round(n_upd * steps / t_batch / 1e9, 3),
             "autotuned": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in tuned.items()},

@@ -1126,6 +1126,11 @@ struct seis3d_batch {
     seis3d::DevStep* d_steps = nullptr;
     float* d_spacing = nullptr;
     seis3d::ShotSample* d_samples = nullptr;
+    // Work-queue mode: every shot's tile map, the shots' first item numbers, and eight queue
+    // counters per launch (zeroed by the graph's first node).
+    seis3d::TileMap* d_maps = nullptr;
+    int* d_first = nullptr;
+    unsigned* d_counters = nullptr;
 
     void release() noexcept {
         if (origin) (void)hipStreamSynchronize(origin);
@@ -1136,7 +1141,9 @@ struct seis3d_batch {
         exec = nullptr; graph = nullptr; origin = nullptr;
         events.clear();
         (void)hipFree(d_steps); (void)hipFree(d_spacing); (void)hipFree(d_samples);
+        (void)hipFree(d_maps); (void)hipFree(d_first); (void)hipFree(d_counters);
         d_steps = nullptr; d_spacing = nullptr; d_samples = nullptr;
+        d_maps = nullptr; d_first = nullptr; d_counters = nullptr;
         if (holds) {
             for (auto* t : sessions) {
                 if (--t->batch_refs == 0 && t->destroy_pending) {
@@ -1351,6 +1358,100 @@ void capture_multi_shot(seis3d_batch* b, const seis3d_source* sources, int64_t n
     check(hipStreamEndCapture(b->origin, &b->graph), "end capture");
 }
 
+// Captures every step of every shot as one work-queue launch per half-step (plus one velocity-ghost
+// and one receiver launch per step for all shots) on b->origin.  The step table holds each shot's
+// parameters per half-step, as in the shared-launch mode; the gather bookkeeping advances as
+// maybe_sample would.
+void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_steps) {
+    const int S = int(b->sessions.size());
+    std::vector<seis3d::DevStep> table(std::size_t(2 * n_steps) * S);
+    std::vector<seis3d::TileMap> maps(S);
+    std::vector<int> first(std::size_t(S) + 1, 0);
+    std::vector<float> spacing(S);
+    int max_gx = 8, max_gy = 8, max_rec = 0;
+    bool any_free = false;
+    for (int s = 0; s < S; ++s) {
+        seis3d_t* t = b->sessions[s];
+        // 64x4 tiles of 16 planes for every shot: items of equal work, so that equal shares of
+        // the item list are equal shares of the work for the eight XCDs.
+        maps[s] = seis3d::dma_tile_map(t->base().g, 4, std::min(seis3d::kQueueChunk, std::max(1, t->c.gz - 8)));
+        first[std::size_t(s) + 1] = first[std::size_t(s)] + maps[s].nblocks;
+        spacing[s] = t->c.spacing;
+        max_gx = std::max(max_gx, t->c.gx);
+        max_gy = std::max(max_gy, t->c.gy);
+        max_rec = std::max(max_rec, t->receiver_count);
+        any_free = any_free || t->c.free_surface;
+    }
+    const int total = first[std::size_t(S)];
+    for (int64_t n = 0; n < n_steps; ++n) {
+        for (int s = 0; s < S; ++s) {
+            seis3d_t* t = b->sessions[s];
+            const seis3d_ctx& c = t->c;
+            seis3d::DevStep V = t->base(), St = V;
+            if (sources[s].kind == SEIS3D_FORCE) V.src = make_source(c, sources[s], double(n) * c.dt);
+            if (sources[s].kind == SEIS3D_EXPLOSION) St.src = make_source(c, sources[s], (double(n) + 0.5) * c.dt);
+            table[std::size_t(2 * n) * S + s] = V;
+            table[std::size_t(2 * n + 1) * S + s] = St;
+        }
+    }
+    std::vector<seis3d::ShotSample> samples;
+    std::vector<std::size_t> start(std::size_t(n_steps) + 1, 0);
+    for (int64_t n = 0; n < n_steps; ++n) {
+        for (int s = 0; s < S; ++s) {
+            seis3d_t* t = b->sessions[s];
+            if (t->recording && t->gather_count < t->gather_target && (std::uint64_t(n) + 1) % t->gather_stride == 0) {
+                samples.push_back({s, t->receiver_count, t->receivers,
+                                   t->gather + std::size_t(t->gather_count) * t->gather_receivers * 3});
+                if (++t->gather_count >= t->gather_target) t->recording = false;
+            }
+        }
+        start[std::size_t(n) + 1] = samples.size();
+    }
+    int blocks = 0;
+    check(seis3d::queue_grid(&blocks), "size work-queue grid");
+    const std::size_t ncounters = std::size_t(n_steps) * 16;
+    check(hipMalloc(&b->d_steps, table.size() * sizeof(seis3d::DevStep)), "allocate batch tables");
+    check(hipMalloc(&b->d_spacing, spacing.size() * sizeof(float)), "allocate batch tables");
+    check(hipMalloc(&b->d_maps, maps.size() * sizeof(seis3d::TileMap)), "allocate batch tables");
+    check(hipMalloc(&b->d_first, first.size() * sizeof(int)), "allocate batch tables");
+    check(hipMalloc(&b->d_counters, ncounters * sizeof(unsigned)), "allocate batch tables");
+    if (!samples.empty())
+        check(hipMalloc(&b->d_samples, samples.size() * sizeof(seis3d::ShotSample)), "allocate batch tables");
+    // The tables go up before capture starts (a synchronous copy may not be captured).
+    check(hipMemcpy(b->d_steps, table.data(), table.size() * sizeof(seis3d::DevStep), hipMemcpyHostToDevice),
+          "upload batch tables");
+    check(hipMemcpy(b->d_spacing, spacing.data(), spacing.size() * sizeof(float), hipMemcpyHostToDevice),
+          "upload batch tables");
+    check(hipMemcpy(b->d_maps, maps.data(), maps.size() * sizeof(seis3d::TileMap), hipMemcpyHostToDevice),
+          "upload batch tables");
+    check(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice),
+          "upload batch tables");
+    if (!samples.empty())
+        check(hipMemcpy(b->d_samples, samples.data(), samples.size() * sizeof(seis3d::ShotSample),
+                        hipMemcpyHostToDevice),
+              "upload batch tables");
+    hipStream_t st = b->origin;
+    check(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal), "begin capture");
+    // Every replay starts with fresh queues.
+    check(seis3d::launch_zero_words(b->d_counters, (long long)ncounters, st), "kernel launch");
+    for (auto* t : b->sessions)
+        if (t->c.free_surface) check(seis3d::launch_stress_ghost(t->base(), st), "kernel launch");
+    for (int64_t n = 0; n < n_steps; ++n) {
+        const seis3d::DevStep* vel = b->d_steps + std::size_t(2 * n) * S;
+        unsigned* q = b->d_counters + std::size_t(n) * 16;
+        check(seis3d::launch_dma_queue(true, vel, b->d_maps, b->d_first, q, S, total, blocks, st), "kernel launch");
+        if (any_free)
+            check(seis3d::launch_velocity_ghost_shots(vel, b->d_spacing, S, max_gx, max_gy, st), "kernel launch");
+        check(seis3d::launch_dma_queue(false, vel + S, b->d_maps, b->d_first, q + 8, S, total, blocks, st),
+              "kernel launch");
+        const std::size_t k = start[std::size_t(n) + 1] - start[std::size_t(n)];
+        if (k)
+            check(seis3d::launch_receivers_shots(vel, b->d_samples + start[std::size_t(n)], int(k), max_rec, st),
+                  "kernel launch");
+    }
+    check(hipStreamEndCapture(st, &b->graph), "end capture");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1363,7 +1464,8 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
 int seis3d_batch_create_mode(seis3d_t* const* sessions, const seis3d_source* sources, int count, int64_t n_steps,
                              int mode, seis3d_batch_t** out) {
     return guarded([&] {
-        if (mode != SEIS3D_BATCH_SHOT_STREAMS && mode != SEIS3D_BATCH_SHARED_LAUNCHES)
+        if (mode != SEIS3D_BATCH_SHOT_STREAMS && mode != SEIS3D_BATCH_SHARED_LAUNCHES &&
+            mode != SEIS3D_BATCH_WORK_QUEUE)
             invalid("Seis3D: unknown batch mode.");
         if (!sessions || !sources || !out || count < 1 || count > 256 || n_steps < 1 || n_steps > 1000000)
             invalid("Seis3D: invalid batch request.");
@@ -1395,10 +1497,12 @@ int seis3d_batch_create_mode(seis3d_t* const* sessions, const seis3d_source* sou
             }
             // Each shot starts by rebuilding its stress ghost planes, so a launch is valid
             // whatever was written into the wavefield since the last one.
-            b->multi = mode == SEIS3D_BATCH_SHARED_LAUNCHES && multi_shot_ok(b->sessions);
+            b->multi = mode != SEIS3D_BATCH_SHOT_STREAMS && multi_shot_ok(b->sessions);
             if (b->multi) {
-                // One launch per half-step for all shots: the shots' tiles share the GPU.
-                capture_multi_shot(b, sources, n_steps);
+                // One launch per half-step for a group of shots or for all of them: the shots'
+                // tiles share the GPU.
+                if (mode == SEIS3D_BATCH_WORK_QUEUE) capture_queue(b, sources, n_steps);
+                else capture_multi_shot(b, sources, n_steps);
                 for (int s = 0; s < count; ++s) {
                     seis3d_t* t = b->sessions[s];
                     b->after[std::size_t(s)] = {t->gather_count, t->recording, true};

@@ -49,8 +49,20 @@ __device__ __forceinline__ void wait_vm() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// Every array the kernels address lives in global memory.  Parameters passed as a kernel argument
+// carry that knowledge already; pointers read from a device table (the work-queue kernels, TABLE)
+// are retyped as global here, so the tile bodies still use global (not flat) instructions.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <bool TABLE, typename T>
+__device__ __forceinline__ auto gptr(T* p) {
+    if constexpr (TABLE) return (global_ptr<T>)p;
+    else return p;
+}
+
 // Lane l of the wave copies src (its own address) to lds_row[l].
-__device__ __forceinline__ void copy_row(const float* src, float* lds_row) {
+template <class F>
+__device__ __forceinline__ void copy_row(F src, float* lds_row) {
     __builtin_amdgcn_global_load_lds(src, lds_row, 4, 0, 0);
 }
 
@@ -66,8 +78,9 @@ struct TileCopy {
     static constexpr int COPIES = PER * (W == 72 ? 2 : 1);
     static_assert(ROWS % BY == 0, "rows per wave must be exact");
 
-    __device__ __forceinline__ static void issue(float* lds, const float* f, const DevGrid& g, int xs, int ys, int L,
-                                                 int tx, int ty) {
+    template <class F>   // const float*, or the same pointer typed as global memory
+    __device__ __forceinline__ static void issue(float* lds, F f, const DevGrid& g, int xs, int ys, int L, int tx,
+                                                 int ty) {
         const long long base = static_cast<long long>(L) * g.plane;
         const int xm = min(xs + tx, g.gx - 1);
         const int xt = min(xs + 8 + tx, g.gx - 1);
@@ -93,13 +106,13 @@ __device__ __forceinline__ void wait_for_ring(bool next_full) {
 // Arrays a half-step writes: its inputs for the in-place step, the output arrays for
 // an out-of-place step.  The choice is a template parameter, so the in-place kernels
 // carry no second set of pointers (and no extra registers for them).
-template <bool OOP>
-__device__ __forceinline__ float* out_field(const DevStep& P, int c) {
-    return OOP ? P.p.out[c] : P.p.f[c];
+template <bool OOP, bool TABLE>
+__device__ __forceinline__ auto out_field(const DevStep& P, int c) {
+    return gptr<TABLE>(OOP ? P.p.out[c] : P.p.f[c]);
 }
-template <bool OOP>
-__device__ __forceinline__ float* out_psi(const DevStep& P, int a) {
-    return OOP ? P.p.psi_out[a] : P.p.psi[a];
+template <bool OOP, bool TABLE>
+__device__ __forceinline__ auto out_psi(const DevStep& P, int a) {
+    return gptr<TABLE>(OOP ? P.p.psi_out[a] : P.p.psi[a]);
 }
 
 template <int BY>
@@ -126,9 +139,11 @@ struct VelOperands {
 };
 
 constexpr int kMaxChunk = 128;   // planes per tile (z table capacity)
+static_assert(kQueueChunk <= kMaxChunk, "work-queue z-chunks fit the z table");
 
 // Copy the tile's slices of the three CPML tables ([half][n][a,b,1/kappa])
 // into LDS as [half*3 + k][index]; entries outside a slab are never read.
+template <bool TABLE>
 __device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, int z0, int z1, int by_rows,
                                             float* cx, float* cyt, float* czt) {
     const DevGrid& g = P.g;
@@ -136,16 +151,16 @@ __device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, in
     const int tid = threadIdx.y * 64 + threadIdx.x, nt = 64 * by_rows;
     for (int k = tid; k < 6 * 64; k += nt) {
         const int j = k / 64, c = min(x0 + k % 64, g.gx - 1);
-        cx[k] = P.p.coef[0][((j / 3) * g.gx + c) * 3 + j % 3];
+        cx[k] = gptr<TABLE>(P.p.coef[0])[((j / 3) * g.gx + c) * 3 + j % 3];
     }
     for (int k = tid; k < 6 * by_rows; k += nt) {
         const int j = k / by_rows, c = min(y0 + k % by_rows, g.gy - 1);
-        cyt[k] = P.p.coef[1][((j / 3) * g.gy + c) * 3 + j % 3];
+        cyt[k] = gptr<TABLE>(P.p.coef[1])[((j / 3) * g.gy + c) * 3 + j % 3];
     }
     const int nz = z1 - z0;
     for (int k = tid; k < 6 * nz; k += nt) {
         const int j = k / nz, c = z0 + k % nz;
-        czt[j * kMaxChunk + k % nz] = P.p.coef[2][((j / 3) * g.gz + c) * 3 + j % 3];
+        czt[j * kMaxChunk + k % nz] = gptr<TABLE>(P.p.coef[2])[((j / 3) * g.gz + c) * 3 + j % 3];
     }
 }
 
@@ -161,7 +176,7 @@ struct VelShared {
     float czt[6 * kMaxChunk];  // z CPML table: [half*3 + k][plane - z0]
 };
 
-template <int BY, bool OOP>
+template <int BY, bool OOP, bool TABLE = false>
 __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
                                               int bz) {
     VelTiles<BY>* ring = sh.ring;
@@ -181,19 +196,19 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
     const int qx = slab_q(x, g.gx, g.pml, false);
     const int qy = slab_q(y, g.gy, g.pml, false);
     const float invh = P.invh;
-    const float* sxx = P.p.f[3];
-    const float* syy = P.p.f[4];
-    const float* szz = P.p.f[5];
-    const float* sxy = P.p.f[6];
-    const float* syz = P.p.f[7];
-    const float* sxz = P.p.f[8];
-    const float* ib = P.p.inv_rho;
+    const auto sxx = gptr<TABLE>(P.p.f[3]);
+    const auto syy = gptr<TABLE>(P.p.f[4]);
+    const auto szz = gptr<TABLE>(P.p.f[5]);
+    const auto sxy = gptr<TABLE>(P.p.f[6]);
+    const auto syz = gptr<TABLE>(P.p.f[7]);
+    const auto sxz = gptr<TABLE>(P.p.f[8]);
+    const auto ib = gptr<TABLE>(P.p.inv_rho);
     constexpr int NDMA = vel_copies<BY>();
     constexpr int NLITE = 2 * TileCopy<BY, BY, 64>::COPIES;
 
     // CPML coefficient tables of this tile (read from LDS: no vector-memory
     // waits inside the plane loop).
-    fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
+    fill_tables<TABLE>(P, x0, y0, z0, z1, BY, cx, cyt, czt);
     __syncthreads();
 
     // Planes z0..z1-1 need whole tiles (x/y derivatives); the halo planes
@@ -219,12 +234,12 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
         if (active && z >= z0) {
             const long long i = dcell(g, x, y, z);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) u.v[c] = P.p.f[c][i];
+            for (int c = 0; c < 3; ++c) u.v[c] = gptr<TABLE>(P.p.f[c])[i];
             u.b[0] = ib[i]; u.b[1] = ib[i + 1]; u.b[2] = ib[i + g.px]; u.b[3] = ib[i + g.plane];
             const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
             if (qz >= 0) {
 #pragma unroll
-                for (int s = 0; s < 3; ++s) u.pz[s] = P.p.psi[2][psi_index_z(g, s, qz, x, y)];
+                for (int s = 0; s < 3; ++s) u.pz[s] = gptr<TABLE>(P.p.psi[2])[psi_index_z(g, s, qz, x, y)];
             }
         }
     };
@@ -253,11 +268,11 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
         if (active && k >= z0 && k < z1) {
             if (qx >= 0) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) px[c] = P.p.psi[0][psi_index_x(g, c, qx, y, k)];
+                for (int c = 0; c < 3; ++c) px[c] = gptr<TABLE>(P.p.psi[0])[psi_index_x(g, c, qx, y, k)];
             }
             if (qy >= 0) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) py[c] = P.p.psi[1][psi_index_y(g, c, qy, x, k)];
+                for (int c = 0; c < 3; ++c) py[c] = gptr<TABLE>(P.p.psi[1])[psi_index_y(g, c, qy, x, k)];
             }
         }
 
@@ -281,7 +296,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             float dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[0] + c0[0] * dz;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 0, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 0, qz, x, y)] = m;
                 dz = c0[2] * dz + m;
             }
             const float div_x = p_vx[0] + dz;
@@ -290,7 +305,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[1] + c0[0] * dz;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 1, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 1, qz, x, y)] = m;
                 dz = c0[2] * dz + m;
             }
             const float div_y = p_vy[0] + dz;
@@ -299,7 +314,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             dz = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[2] + c1[0] * dz;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 2, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 2, qz, x, y)] = m;
                 dz = c1[2] * dz + m;
             }
             const float div_z = p_vz[0] + dz;
@@ -316,7 +331,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
                 }
             }
             const long long i = dcell(g, x, y, z);
-            out_field<OOP>(P, 0)[i] = vx; out_field<OOP>(P, 1)[i] = vy; out_field<OOP>(P, 2)[i] = vz;
+            out_field<OOP, TABLE>(P, 0)[i] = vx; out_field<OOP, TABLE>(P, 1)[i] = vy; out_field<OOP, TABLE>(P, 2)[i] = vz;
             bad |= !isfinite(vx) || !isfinite(vy) || !isfinite(vz);
         }
         float pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
@@ -328,7 +343,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             float d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[0] + cx[3 * 64 + tx] * d;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 0, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 0, qx, y, k)] = m;
                 d = cx[5 * 64 + tx] * d + m;
             }
             pvx += d;
@@ -337,7 +352,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[0] + cyt[0 * BY + ty] * d;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 0, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 0, qy, x, k)] = m;
                 d = cyt[2 * BY + ty] * d + m;
             }
             pvx += d;
@@ -347,7 +362,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[1] + cx[0 * 64 + tx] * d;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 1, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 1, qx, y, k)] = m;
                 d = cx[2 * 64 + tx] * d + m;
             }
             pvy += d;
@@ -356,7 +371,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[1] + cyt[3 * BY + ty] * d;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 1, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 1, qy, x, k)] = m;
                 d = cyt[5 * BY + ty] * d + m;
             }
             pvy += d;
@@ -366,7 +381,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             d = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[2] + cx[0 * 64 + tx] * d;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 2, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 2, qx, y, k)] = m;
                 d = cx[2 * 64 + tx] * d + m;
             }
             pvz += d;
@@ -375,7 +390,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& 
             d = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[2] + cyt[0 * BY + ty] * d;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 2, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 2, qy, x, k)] = m;
                 d = cyt[2 * BY + ty] * d + m;
             }
             pvz += d;
@@ -430,7 +445,7 @@ struct StrShared {
     float czt[6 * kMaxChunk];
 };
 
-template <int BY, bool OOP>
+template <int BY, bool OOP, bool TABLE = false>
 __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
                                             int bz) {
     StrTiles<BY>* ring = sh.ring;
@@ -454,7 +469,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
 
     // CPML coefficient tables of this tile (read from LDS: no vector-memory
     // waits inside the plane loop).
-    fill_tables(P, x0, y0, z0, z1, BY, cx, cyt, czt);
+    fill_tables<TABLE>(P, x0, y0, z0, z1, BY, cx, cyt, czt);
     __syncthreads();
 
     // Halo planes only feed the z queues: copy just the tile's own rows.
@@ -462,14 +477,14 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
     auto copy_plane = [&](int slot, int L) {
         StrTiles<BY>& s = ring[(slot - k0) % 3];
         if (full_plane(slot)) {
-            TileCopy<BY, BY + 8, 72>::issue(s.vx, P.p.f[0], g, x0 - 4, y0 - 4, L, tx, ty);
-            TileCopy<BY, BY + 8, 72>::issue(s.vy, P.p.f[1], g, x0 - 4, y0 - 4, L, tx, ty);
-            TileCopy<BY, BY + 8, 72>::issue(s.vz, P.p.f[2], g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.vx, gptr<TABLE>(P.p.f[0]), g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.vy, gptr<TABLE>(P.p.f[1]), g, x0 - 4, y0 - 4, L, tx, ty);
+            TileCopy<BY, BY + 8, 72>::issue(s.vz, gptr<TABLE>(P.p.f[2]), g, x0 - 4, y0 - 4, L, tx, ty);
         } else {
             constexpr int own = 4 * 72 + 4;
-            TileCopy<BY, BY, 64, 72>::issue(s.vx + own, P.p.f[0], g, x0, y0, L, tx, ty);
-            TileCopy<BY, BY, 64, 72>::issue(s.vy + own, P.p.f[1], g, x0, y0, L, tx, ty);
-            TileCopy<BY, BY, 64, 72>::issue(s.vz + own, P.p.f[2], g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64, 72>::issue(s.vx + own, gptr<TABLE>(P.p.f[0]), g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64, 72>::issue(s.vy + own, gptr<TABLE>(P.p.f[1]), g, x0, y0, L, tx, ty);
+            TileCopy<BY, BY, 64, 72>::issue(s.vz + own, gptr<TABLE>(P.p.f[2]), g, x0, y0, L, tx, ty);
         }
     };
     auto load_operands = [&](int k, StrOperands& u) {
@@ -477,19 +492,19 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
         if (active && z >= z0) {
             const long long i = dcell(g, x, y, z);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) u.o[c] = P.p.f[3 + c][i];
-            u.lam = P.p.lambda[i];
-            u.mu = P.p.mu[i];
+            for (int c = 0; c < 6; ++c) u.o[c] = gptr<TABLE>(P.p.f[3 + c])[i];
+            u.lam = gptr<TABLE>(P.p.lambda)[i];
+            u.mu = gptr<TABLE>(P.p.mu)[i];
             // Shear moduli around the cell: the harmonic means below divide by them
             // exactly as shearMu3D does (no 1/mu array to read).
-            const float* r = P.p.mu;
+            const auto r = gptr<TABLE>(P.p.mu);
             const long long sy = g.px, sz = g.plane;
             u.r[1] = r[i + 1]; u.r[2] = r[i + sy]; u.r[3] = r[i + sz];
             u.r[4] = r[i + 1 + sy]; u.r[5] = r[i + sy + sz]; u.r[6] = r[i + 1 + sz];
             const int qz = slab_q(z, g.gz, g.pml, g.free_surface != 0);
             if (qz >= 0) {
 #pragma unroll
-                for (int s = 0; s < 3; ++s) u.pz[s] = P.p.psi[2][psi_index_z(g, 3 + s, qz, x, y)];
+                for (int s = 0; s < 3; ++s) u.pz[s] = gptr<TABLE>(P.p.psi[2])[psi_index_z(g, 3 + s, qz, x, y)];
             }
         }
     };
@@ -518,11 +533,11 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
         if (active && k >= z0 && k < z1) {
             if (qx >= 0) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) px[j] = P.p.psi[0][psi_index_x(g, 3 + j, qx, y, k)];
+                for (int j = 0; j < 3; ++j) px[j] = gptr<TABLE>(P.p.psi[0])[psi_index_x(g, 3 + j, qx, y, k)];
             }
             if (qy >= 0) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) py[j] = P.p.psi[1][psi_index_y(g, 3 + j, qy, x, k)];
+                for (int j = 0; j < 3; ++j) py[j] = gptr<TABLE>(P.p.psi[1])[psi_index_y(g, 3 + j, qy, x, k)];
             }
         }
         const int c = (ty + 4) * 72 + tx + 4;
@@ -543,7 +558,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             float d02 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[0] + c1[0] * d02;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 3, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 3, qz, x, y)] = m;
                 d02 = c1[2] * d02 + m;
             }
 #pragma unroll
@@ -551,7 +566,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             float d12 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c1[1] * cur.pz[1] + c1[0] * d12;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 4, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 4, qz, x, y)] = m;
                 d12 = c1[2] * d12 + m;
             }
 #pragma unroll
@@ -559,7 +574,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             float d22 = stencil8(w, invh);
             if (qz >= 0) {
                 const float m = c0[1] * cur.pz[2] + c0[0] * d22;
-                out_psi<OOP>(P, 2)[psi_index_z(g, 5, qz, x, y)] = m;
+                out_psi<OOP, TABLE>(P, 2)[psi_index_z(g, 5, qz, x, y)] = m;
                 d22 = c0[2] * d22 + m;
             }
             const bool surface = g.free_surface && z == 4;
@@ -592,8 +607,8 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
                 }
             }
             const long long i = dcell(g, x, y, z);
-            out_field<OOP>(P, 3)[i] = sxx; out_field<OOP>(P, 4)[i] = syy; out_field<OOP>(P, 5)[i] = szz;
-            out_field<OOP>(P, 6)[i] = sxy; out_field<OOP>(P, 7)[i] = syz; out_field<OOP>(P, 8)[i] = sxz;
+            out_field<OOP, TABLE>(P, 3)[i] = sxx; out_field<OOP, TABLE>(P, 4)[i] = syy; out_field<OOP, TABLE>(P, 5)[i] = szz;
+            out_field<OOP, TABLE>(P, 6)[i] = sxy; out_field<OOP, TABLE>(P, 7)[i] = syz; out_field<OOP, TABLE>(P, 8)[i] = sxz;
             bad |= !isfinite(sxx) || !isfinite(syy) || !isfinite(szz) || !isfinite(sxy) || !isfinite(syz) ||
                    !isfinite(sxz);
             // Free-surface ghost planes (stressGhostKernel3D, cuda/Cuda_Seis3D.cu:203-219) from
@@ -602,11 +617,11 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             if (g.free_surface && z <= 8) {
                 if (z >= 5) {
                     const long long gi = dcell(g, x, y, 8 - z);
-                    out_field<OOP>(P, 5)[gi] = -szz; out_field<OOP>(P, 3)[gi] = sxx; out_field<OOP>(P, 4)[gi] = syy; out_field<OOP>(P, 6)[gi] = sxy;
+                    out_field<OOP, TABLE>(P, 5)[gi] = -szz; out_field<OOP, TABLE>(P, 3)[gi] = sxx; out_field<OOP, TABLE>(P, 4)[gi] = syy; out_field<OOP, TABLE>(P, 6)[gi] = sxy;
                 }
                 if (z <= 7) {
                     const long long gi = dcell(g, x, y, 7 - z);
-                    out_field<OOP>(P, 8)[gi] = -sxz; out_field<OOP>(P, 7)[gi] = -syz;
+                    out_field<OOP, TABLE>(P, 8)[gi] = -sxz; out_field<OOP, TABLE>(P, 7)[gi] = -syz;
                 }
             }
         }
@@ -619,7 +634,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             d00 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[1 * 64 + tx] * px[0] + cx[0 * 64 + tx] * d00;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 3, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 3, qx, y, k)] = m;
                 d00 = cx[2 * 64 + tx] * d00 + m;
             }
 #pragma unroll
@@ -627,7 +642,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             float d01 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[0] + cyt[3 * BY + ty] * d01;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 3, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 3, qy, x, k)] = m;
                 d01 = cyt[5 * BY + ty] * d01 + m;
             }
             // vy: D+x (slot 4, half), D-y (slot 4, integer).
@@ -636,7 +651,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             float d10 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[1] + cx[3 * 64 + tx] * d10;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 4, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 4, qx, y, k)] = m;
                 d10 = cx[5 * 64 + tx] * d10 + m;
             }
 #pragma unroll
@@ -644,7 +659,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             d11 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[1 * BY + ty] * py[1] + cyt[0 * BY + ty] * d11;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 4, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 4, qy, x, k)] = m;
                 d11 = cyt[2 * BY + ty] * d11 + m;
             }
             s01 = d01 + d10;
@@ -654,7 +669,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             d20 = stencil8(w, invh);
             if (qx >= 0) {
                 const float m = cx[4 * 64 + tx] * px[2] + cx[3 * 64 + tx] * d20;
-                out_psi<OOP>(P, 0)[psi_index_x(g, 5, qx, y, k)] = m;
+                out_psi<OOP, TABLE>(P, 0)[psi_index_x(g, 5, qx, y, k)] = m;
                 d20 = cx[5 * 64 + tx] * d20 + m;
             }
 #pragma unroll
@@ -662,7 +677,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P,
             d21 = stencil8(w, invh);
             if (qy >= 0) {
                 const float m = cyt[4 * BY + ty] * py[2] + cyt[3 * BY + ty] * d21;
-                out_psi<OOP>(P, 1)[psi_index_y(g, 5, qy, x, k)] = m;
+                out_psi<OOP, TABLE>(P, 1)[psi_index_y(g, 5, qy, x, k)] = m;
                 d21 = cyt[5 * BY + ty] * d21 + m;
             }
         }
@@ -748,6 +763,94 @@ __global__ __launch_bounds__(64 * BY) void stress_dma_shots(ShotLaunch L) {
     int s, bx, by, bz;
     if (!shot_tile_of(L, s, bx, by, bz)) return;
     stress_shot<BY>(sh, L, s, bx, by, bz);
+}
+
+// Work-queue half-steps: a persistent grid (every workgroup resident) pulls (shot, tile, z-chunk)
+// items from per-XCD queues until they run dry.  The items of all shots form one list (shot s owns
+// items [first[s], first[s + 1])), dealt to the eight XCDs in contiguous eighths so that
+// neighbouring tiles share an XCD's L2, as tile_of deals them; within its eighth an XCD's
+// workgroups take the next item from the XCD's counter, so shots of uneven size balance.
+// counters[0..7] must be zero at launch (one set per launch).  Every workgroup leaves its loop
+// as soon as its XCD's counter passes the end of the XCD's range, so the grid always drains.
+__device__ __forceinline__ bool queue_item(const TileMap* __restrict__ maps, const int* __restrict__ first,
+                                           unsigned* counters, int nshots, int total, int* slot, int& s, int& bx,
+                                           int& by, int& bz) {
+    const int xcd = int(blockIdx.x) & 7;
+    const int lo = int((static_cast<long long>(xcd) * total) / 8);
+    const int hi = int((static_cast<long long>(xcd + 1) * total) / 8);
+    __syncthreads();   // every wave has finished with the previous item (its LDS and *slot)
+    if (threadIdx.x == 0 && threadIdx.y == 0) *slot = lo + int(atomicAdd(&counters[xcd], 1u));
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(*slot);
+    if (item >= hi) return false;
+    int k = 0;
+    while (k + 1 < nshots && first[k + 1] <= item) ++k;
+    s = __builtin_amdgcn_readfirstlane(k);
+    const TileMap& m = maps[s];
+    const int t = item - first[s];
+    bx = __builtin_amdgcn_readfirstlane(t % m.nbx);
+    by = __builtin_amdgcn_readfirstlane((t / m.nbx) % m.nby);
+    bz = __builtin_amdgcn_readfirstlane(t / (m.nbx * m.nby));
+    return true;
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma_queue(
+    const DevStep* __restrict__ steps, const TileMap* __restrict__ maps, const int* __restrict__ first,
+    unsigned* counters, int nshots, int total) {
+    __shared__ VelShared<BY> sh;
+    __shared__ int slot;
+    int s, bx, by, bz;
+    while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
+        velocity_tile<BY, false, true>(sh, steps[s], maps[s], bx, by, bz);
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void stress_dma_queue(const DevStep* __restrict__ steps,
+                                                          const TileMap* __restrict__ maps,
+                                                          const int* __restrict__ first, unsigned* counters,
+                                                          int nshots, int total) {
+    __shared__ StrShared<BY> sh;
+    __shared__ int slot;
+    int s, bx, by, bz;
+    while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
+        stress_tile<BY, false, true>(sh, steps[s], maps[s], bx, by, bz);
+}
+
+__global__ void zero_words(unsigned* w, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0u;
+}
+
+hipError_t launch_zero_words(unsigned* w, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_words, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, w, n);
+    return hipGetLastError();
+}
+
+hipError_t queue_grid(int* blocks) {
+    int dev = 0, cus = 0, v = 0, st = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma_queue<4>, 64 * 4, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma_queue<4>, 64 * 4, 0);
+    if (e != hipSuccess) return e;
+    // Whole rounds of eight; a workgroup that finds no resident slot starts when another leaves and
+    // finds the queues empty, so the count only has to cover the resident slots.
+    *blocks = std::max(8, (std::max(v, st) * cus) / 8 * 8);
+    return hipSuccess;
+}
+
+hipError_t launch_dma_queue(bool velocity, const DevStep* steps, const TileMap* maps, const int* first,
+                            unsigned* counters, int nshots, int total, int blocks, hipStream_t s) {
+    if (nshots < 1 || total < 1 || blocks < 8 || blocks % 8) return hipErrorInvalidValue;
+    if (velocity)
+        hipLaunchKernelGGL(velocity_dma_queue<4>, dim3(unsigned(blocks)), dim3(64, 4), 0, s, steps, maps, first,
+                           counters, nshots, total);
+    else
+        hipLaunchKernelGGL(stress_dma_queue<4>, dim3(unsigned(blocks)), dim3(64, 4), 0, s, steps, maps, first,
+                           counters, nshots, total);
+    return hipGetLastError();
 }
 
 hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {

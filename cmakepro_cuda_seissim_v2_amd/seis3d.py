@@ -638,14 +638,22 @@ class Batch:
     created; after launch() + synchronize() every session's gather holds its shot record.
     """
 
-    def __init__(self, sessions, sources, n_steps: int, shared_launches: bool = False):
+    MODES = {"shot_streams": 0, "shared_launches": 1, "work_queue": 2}
+
+    def __init__(self, sessions, sources, n_steps: int, shared_launches: bool = False, mode: str | None = None):
+        """mode: "shot_streams" (a stream per shot, the default), "shared_launches" (groups of up to
+        four shots per launch) or "work_queue" (one persistent launch per half-step for all shots);
+        shared_launches=True is the same as mode="shared_launches"."""
         self.sessions = list(sessions)
         n = len(self.sessions)
+        if mode is None:
+            mode = "shared_launches" if shared_launches else "shot_streams"
+        if mode not in self.MODES:
+            raise ValueError(f"unknown batch mode {mode!r}")
         hs = (C.c_void_p * n)(*[g.handle.value for g in self.sessions])
         srcs = (Source * n)(*sources)
         self._b = C.c_void_p()
-        _check(library().seis3d_batch_create_mode(hs, srcs, n, int(n_steps), 1 if shared_launches else 0,
-                                                  C.byref(self._b)))
+        _check(library().seis3d_batch_create_mode(hs, srcs, n, int(n_steps), self.MODES[mode], C.byref(self._b)))
 
     def launch(self) -> None:
         _check(library().seis3d_batch_launch(self._b))

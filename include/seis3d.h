@@ -291,9 +291,16 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
 /* Batch modes.  SHOT_STREAMS (what seis3d_batch_create builds): every shot's own launches on its
  * own stream.  SHARED_LAUNCHES: shots grouped by tile height, up to 4 per group, each group one
  * launch per half-step for all its shots (the shots' tiles dealt out as one list) on its own
- * stream; shots with reflecting walls or a learned frame make the batch fall back to
- * SHOT_STREAMS.  Both give every shot's gather bit for bit as the shot run alone. */
-enum seis3d_batch_mode { SEIS3D_BATCH_SHOT_STREAMS = 0, SEIS3D_BATCH_SHARED_LAUNCHES = 1 };
+ * stream.  WORK_QUEUE: one persistent launch per half-step for every shot of the batch: a grid of
+ * resident 64x4 workgroups pulls (shot, tile, z-chunk) items from per-XCD queues, so shots of
+ * uneven size share the GPU without a tail per shot; each shot is stepped in 64x4x16 items.
+ * Shots with reflecting walls or a learned frame make the last two fall back to SHOT_STREAMS.
+ * Every mode gives every shot's gather bit for bit as the shot run alone. */
+enum seis3d_batch_mode {
+    SEIS3D_BATCH_SHOT_STREAMS = 0,
+    SEIS3D_BATCH_SHARED_LAUNCHES = 1,
+    SEIS3D_BATCH_WORK_QUEUE = 2
+};
 int seis3d_batch_create_mode(seis3d_t* const* sessions, const seis3d_source* sources, int count,
                              int64_t n_steps, int mode, seis3d_batch_t** out);
 int seis3d_batch_launch(seis3d_batch_t* b);

@@ -35,8 +35,8 @@ def plain_shot(ctx, model, src, rec, steps, stride):
     return np.stack(samples)
 
 
-@pytest.mark.parametrize("shared", [False, True], ids=["shot-streams", "shared-launches"])
-def test_batched_shots_equal_single_shots(gpu, shared):
+@pytest.mark.parametrize("mode", ["shot_streams", "shared_launches", "work_queue"])
+def test_batched_shots_equal_single_shots(gpu, mode):
     shots = W.batch_shots(5, seed=11, sizes=[(40, 24, 36), (56, 20, 32), (33, 27, 41), (48, 32, 24), (36, 36, 36)],
                           corr=4.0, receivers=20, pml=8)
     steps, stride = 120, 2
@@ -45,7 +45,7 @@ def test_batched_shots_equal_single_shots(gpu, shared):
         g = prepare(ctx, model, rec)
         g.gather_begin(steps // stride, stride)
         sessions.append(g)
-    batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=shared)
+    batch = S.Batch(sessions, [s[2] for s in shots], steps, mode=mode)
     batch.launch()
     batch.synchronize()
     for (ctx, model, src, rec), g in zip(shots, sessions):
@@ -58,20 +58,30 @@ def test_batched_shots_equal_single_shots(gpu, shared):
         g.release()
 
 
-def test_batch_replay_is_deterministic(gpu):
-    ctx, model, src, rec = W.batch_shots(1, seed=3, sizes=[(40, 24, 40)], corr=4.0, receivers=10, pml=8)[0]
-    g = prepare(ctx, model, rec)
-    g.gather_begin(50, 1)
-    batch = S.Batch([g], [src], 50)
+@pytest.mark.parametrize("mode", ["shot_streams", "work_queue"])
+def test_batch_replay_is_deterministic(gpu, mode):
+    """Every replay of the graph restarts from the reset wavefields (and, in the work-queue mode,
+    from freshly zeroed queues), with eager work and host synchronisation between replays."""
+    shots = W.batch_shots(2, seed=3, sizes=[(40, 24, 40), (36, 28, 32)], corr=4.0, receivers=10, pml=8)
+    sessions = []
+    for ctx, model, src, rec in shots:
+        g = prepare(ctx, model, rec)
+        g.gather_begin(50, 1)
+        sessions.append(g)
+    batch = S.Batch(sessions, [s[2] for s in shots], 50, mode=mode)
     out = []
     for _ in range(3):
-        g.reset()
+        for g in sessions:
+            g.reset()
+            g.synchronize()
         batch.launch()
         batch.synchronize()
-        out.append(g.gather_read())
-    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+        out.append([g.gather_read() for g in sessions])
+    for later in out[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(out[0], later))
     batch.close()
-    g.release()
+    for g in sessions:
+        g.release()
 
 
 def test_autotune_choice_is_valid_and_bitwise_neutral(gpu):

@@ -140,34 +140,39 @@ def test_reference_boundary_cases_match_published_ratios(gpu):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("shared", [False, True], ids=["shot-streams", "shared-launches"])
-def test_config3_sixteen_shot_batch_equals_single_shots(gpu, shared):
+def test_config3_sixteen_shot_batch_equals_single_shots(gpu):
+    """Config 3 at full size: 16 shots of 128^3 .. 256x128x256, 800 steps, autotuned tiles.  Each
+    shot is first run alone; then every batch mode must reproduce every gather bit for bit."""
     shots = W.batch_shots(16)
     steps, stride = 800, 2
-    sessions = []
+    sessions, alone = [], []
     for ctx, model, src, rec in shots:
         g = S.GPUSimData3D(ctx)
         g.upload_model(*model)
         g.set_receivers(rec)
         g.autotune(src, steps=2)
         g.gather_begin(steps // stride, stride)
-        sessions.append(g)
-    log("config 3: 16 shots captured")
-    batch = S.Batch(sessions, [s[2] for s in shots], steps, shared_launches=shared)
-    batch.launch()
-    batch.synchronize()
-    batched = [g.gather_read() for g in sessions]
-    batch.close()
-    for k, ((ctx, model, src, rec), g) in enumerate(zip(shots, sessions)):
-        g.reset()
-        g.gather_end()
-        g.gather_begin(steps // stride, stride)
         g.run(src, 0, steps)
-        alone = g.gather_read()
-        assert alone.shape == (steps // stride, len(rec), 3) and np.any(alone)
-        assert np.array_equal(batched[k], alone), (k, float(np.max(np.abs(batched[k] - alone))))
+        a = g.gather_read()
+        assert a.shape == (steps // stride, len(rec), 3) and np.any(a)
+        alone.append(a)
+        sessions.append(g)
+    log("config 3: 16 shots run alone")
+    for mode in ("shot_streams", "shared_launches", "work_queue"):
+        for g in sessions:
+            g.reset()
+            g.gather_end()
+            g.gather_begin(steps // stride, stride)
+        batch = S.Batch(sessions, [s[2] for s in shots], steps, mode=mode)
+        batch.launch()
+        batch.synchronize()
+        batch.close()
+        for k, g in enumerate(sessions):
+            got = g.gather_read()
+            assert np.array_equal(got, alone[k]), (mode, k, float(np.max(np.abs(got - alone[k]))))
+        log(f"config 3: all 16 gathers of the {mode} batch bit-identical")
+    for g in sessions:
         g.release()
-    log("config 3: all 16 gathers bit-identical")
 
 
 def fwi_misfit(ctx, model, shots, observed, steps, stride):
