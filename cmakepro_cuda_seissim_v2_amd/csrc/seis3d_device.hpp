@@ -90,13 +90,16 @@ __device__ __forceinline__ int slab_q(int coord, int n, int pml, bool top_free) 
     return -1;
 }
 
-__device__ __forceinline__ long long psi_index_x(const DevGrid& g, int slot, int q, int y, int z) {
+template <class G>
+__device__ __forceinline__ long long psi_index_x(const G& g, int slot, int q, int y, int z) {
     return ((static_cast<long long>(slot) * g.gz + z) * g.gy + y) * (2 * g.pml) + q;
 }
-__device__ __forceinline__ long long psi_index_y(const DevGrid& g, int slot, int q, int x, int z) {
+template <class G>
+__device__ __forceinline__ long long psi_index_y(const G& g, int slot, int q, int x, int z) {
     return ((static_cast<long long>(slot) * g.gz + z) * (2 * g.pml) + q) * g.px + x;
 }
-__device__ __forceinline__ long long psi_index_z(const DevGrid& g, int slot, int q, int x, int y) {
+template <class G>
+__device__ __forceinline__ long long psi_index_z(const G& g, int slot, int q, int x, int y) {
     return ((static_cast<long long>(slot) * g.tz + q) * g.gy + y) * g.px + x;
 }
 

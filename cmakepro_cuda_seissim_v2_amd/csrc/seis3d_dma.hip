@@ -27,15 +27,18 @@ namespace seis3d {
 
 namespace {
 
-__device__ __forceinline__ long long dcell(const DevGrid& g, int x, int y, int z) {
+template <class G>
+__device__ __forceinline__ long long dcell(const G& g, int x, int y, int z) {
     return (static_cast<long long>(z) * g.gy + y) * g.px + x;
 }
 
-__device__ __forceinline__ float source_weight(const DevSource& s, int r2) {
+template <class S>
+__device__ __forceinline__ float source_weight(const S& s, int r2) {
     return r2 == 0 ? s.w[0] : r2 == 1 ? s.w[1] : r2 == 2 ? s.w[2] : s.w[3];
 }
 
-__device__ __forceinline__ int source_r2d(const DevSource& s, int x, int y, int z) {
+template <class S>
+__device__ __forceinline__ int source_r2d(const S& s, int x, int y, int z) {
     const int dx = x - s.x, dy = y - s.y, dz = z - s.z;
     if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return -1;
     return dx * dx + dy * dy + dz * dz;
@@ -78,8 +81,8 @@ struct TileCopy {
     static constexpr int COPIES = PER * (W == 72 ? 2 : 1);
     static_assert(ROWS % BY == 0, "rows per wave must be exact");
 
-    template <class F>   // const float*, or the same pointer typed as global memory
-    __device__ __forceinline__ static void issue(float* lds, F f, const DevGrid& g, int xs, int ys, int L, int tx,
+    template <class F, class G>   // const float*, or the same pointer typed as global memory
+    __device__ __forceinline__ static void issue(float* lds, F f, const G& g, int xs, int ys, int L, int tx,
                                                  int ty) {
         const long long base = static_cast<long long>(L) * g.plane;
         const int xm = min(xs + tx, g.gx - 1);
@@ -106,12 +109,12 @@ __device__ __forceinline__ void wait_for_ring(bool next_full) {
 // Arrays a half-step writes: its inputs for the in-place step, the output arrays for
 // an out-of-place step.  The choice is a template parameter, so the in-place kernels
 // carry no second set of pointers (and no extra registers for them).
-template <bool OOP, bool TABLE>
-__device__ __forceinline__ auto out_field(const DevStep& P, int c) {
+template <bool OOP, bool TABLE, class Step>
+__device__ __forceinline__ auto out_field(const Step& P, int c) {
     return gptr<TABLE>(OOP ? P.p.out[c] : P.p.f[c]);
 }
-template <bool OOP, bool TABLE>
-__device__ __forceinline__ auto out_psi(const DevStep& P, int a) {
+template <bool OOP, bool TABLE, class Step>
+__device__ __forceinline__ auto out_psi(const Step& P, int a) {
     return gptr<TABLE>(OOP ? P.p.psi_out[a] : P.p.psi[a]);
 }
 
@@ -143,10 +146,10 @@ static_assert(kQueueChunk <= kMaxChunk, "work-queue z-chunks fit the z table");
 
 // Copy the tile's slices of the three CPML tables ([half][n][a,b,1/kappa])
 // into LDS as [half*3 + k][index]; entries outside a slab are never read.
-template <bool TABLE>
-__device__ __forceinline__ void fill_tables(const DevStep& P, int x0, int y0, int z0, int z1, int by_rows,
+template <bool TABLE, class Step>
+__device__ __forceinline__ void fill_tables(const Step& P, int x0, int y0, int z0, int z1, int by_rows,
                                             float* cx, float* cyt, float* czt) {
-    const DevGrid& g = P.g;
+    const auto& g = P.g;
     if (g.pml == 0) return;
     const int tid = threadIdx.y * 64 + threadIdx.x, nt = 64 * by_rows;
     for (int k = tid; k < 6 * 64; k += nt) {
@@ -176,14 +179,14 @@ struct VelShared {
     float czt[6 * kMaxChunk];  // z CPML table: [half*3 + k][plane - z0]
 };
 
-template <int BY, bool OOP, bool TABLE = false>
-__device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
+template <int BY, bool OOP, bool TABLE = false, class Step = DevStep>
+__device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const Step& P, const TileMap& M, int bx, int by,
                                               int bz) {
     VelTiles<BY>* ring = sh.ring;
     float* cx = sh.cx;
     float* cyt = sh.cyt;
     float* czt = sh.czt;
-    const DevGrid& g = P.g;
+    const auto& g = P.g;
     const int tx = threadIdx.x;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
@@ -445,14 +448,14 @@ struct StrShared {
     float czt[6 * kMaxChunk];
 };
 
-template <int BY, bool OOP, bool TABLE = false>
-__device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const DevStep& P, const TileMap& M, int bx, int by,
+template <int BY, bool OOP, bool TABLE = false, class Step = DevStep>
+__device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const Step& P, const TileMap& M, int bx, int by,
                                             int bz) {
     StrTiles<BY>* ring = sh.ring;
     float* cx = sh.cx;
     float* cyt = sh.cyt;
     float* czt = sh.czt;
-    const DevGrid& g = P.g;
+    const auto& g = P.g;
     const int tx = threadIdx.x;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int x0 = 4 + bx * 64, y0 = 4 + by * BY;
@@ -794,6 +797,12 @@ __device__ __forceinline__ bool queue_item(const TileMap* __restrict__ maps, con
     return true;
 }
 
+// The step table read as constant memory (the kernels never write it): the tile bodies then read
+// a shot's parameters with scalar loads they may repeat at will, as from the kernel arguments,
+// instead of holding them in registers (read as ordinary global memory, the stress kernel needed
+// 154 VGPRs and ran 15-20 % slower).
+using ConstStep = const __attribute__((address_space(4))) DevStep;
+
 template <int BY>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma_queue(
     const DevStep* __restrict__ steps, const TileMap* __restrict__ maps, const int* __restrict__ first,
@@ -802,11 +811,11 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
     __shared__ int slot;
     int s, bx, by, bz;
     while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
-        velocity_tile<BY, false, true>(sh, steps[s], maps[s], bx, by, bz);
+        velocity_tile<BY, false, true>(sh, ((ConstStep*)steps)[s], maps[s], bx, by, bz);
 }
 
 template <int BY>
-__global__ __launch_bounds__(64 * BY) void stress_dma_queue(const DevStep* __restrict__ steps,
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void stress_dma_queue(const DevStep* __restrict__ steps,
                                                           const TileMap* __restrict__ maps,
                                                           const int* __restrict__ first, unsigned* counters,
                                                           int nshots, int total) {
@@ -814,7 +823,7 @@ __global__ __launch_bounds__(64 * BY) void stress_dma_queue(const DevStep* __res
     __shared__ int slot;
     int s, bx, by, bz;
     while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
-        stress_tile<BY, false, true>(sh, steps[s], maps[s], bx, by, bz);
+        stress_tile<BY, false, true>(sh, ((ConstStep*)steps)[s], maps[s], bx, by, bz);
 }
 
 __global__ void zero_words(unsigned* w, long long n) {
