@@ -97,3 +97,41 @@ def test_autotune_choice_is_valid_and_bitwise_neutral(gpu):
         assert np.array_equal(g.download_field(f), h.download_field(f))
     g.release()
     h.release()
+
+
+@pytest.mark.parametrize("mode", ["shared_launches", "work_queue"])
+def test_mixed_boundaries_and_sources_equal_single_shots(gpu, mode):
+    """Shots that differ in everything a multi-shot launch carries per shot: CPML thickness, a free
+    top or CPML on all six faces, explosion or oblique force sources, receiver counts."""
+    base = W.batch_shots(4, seed=23, sizes=[(40, 24, 36), (44, 28, 30), (36, 32, 40), (48, 20, 28)], corr=4.0,
+                         receivers=12, pml=8)
+    shots = []
+    for k, (ctx, model, src, rec) in enumerate(base):
+        vp = model[0]
+        c = W.imported_context(ctx.nx, ctx.ny, ctx.nz, max_vp=float(vp.max()), pml=8 + 2 * k,
+                               free_surface=k % 2 == 0)
+        if k % 2:
+            s = S.default_source(x=ctx.nx // 2, y=ctx.ny // 2, z=ctx.nz // 2, kind=S.EXPLOSION, amplitude=1.0e8)
+            r = [(x, ctx.ny // 2, 3) for x in range(2, ctx.nx - 2, 4)]
+        else:
+            s = S.default_source(x=ctx.nx // 3, y=ctx.ny // 2, z=4, kind=S.FORCE, amplitude=1.0e8,
+                                 direction=(0.6, 0.0, 0.8))
+            r = rec
+        shots.append((c, model, s, r))
+    steps, stride = 90, 3
+    sessions = []
+    for ctx, model, src, rec in shots:
+        g = prepare(ctx, model, rec)
+        g.gather_begin(steps // stride, stride)
+        sessions.append(g)
+    batch = S.Batch(sessions, [s[2] for s in shots], steps, mode=mode)
+    batch.launch()
+    batch.synchronize()
+    for (ctx, model, src, rec), g in zip(shots, sessions):
+        got = g.gather_read()
+        ref = plain_shot(ctx, model, src, rec, steps, stride)
+        assert np.any(ref)
+        assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+    batch.close()
+    for g in sessions:
+        g.release()
