@@ -121,10 +121,13 @@ def test_out_of_place_recompute_equals_copy_path(gpu, small, family, tile_y, z_c
         assert np.array_equal(a, b)
 
 
-def test_batched_request_equals_single_shots(gpu):
+@pytest.mark.parametrize("boundary", ["cpml", "free", "walls"])
+def test_batched_request_equals_single_shots(gpu, boundary):
+    """A request with several shots (on their own streams, in lockstep) must equal the sum of the
+    shots requested one at a time, with CPML all round, a free top and reflecting walls."""
     start, true = F.models(NX, NY, NZ)
-    ctx = F.context(NX, NY, NZ, true)
-    rec = F.receivers(NX, NY, NZ)
+    ctx = F.context(NX, NY, NZ, true, free_surface=boundary == "free", reflecting=boundary == "walls")
+    rec = F.receivers(NX, NY, NZ, surface=boundary == "free")
     shots = []
     for k, kind in enumerate((S.EXPLOSION, S.FORCE, S.EXPLOSION)):
         src = F.source(kind, NX, NY, NZ, shift=3 * k - 3)
