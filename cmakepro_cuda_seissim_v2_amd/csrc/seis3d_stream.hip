@@ -1,13 +1,15 @@
 // seis3d_stream.hip - HBM bandwidth probes: the measured ceiling the step kernels are
 // compared with.
 //
-// Every probe streams whole arrays with 16-byte accesses per lane and a grid-stride loop
-// (unrolled by four where the kind allows, so that each wave keeps four 1 KiB requests in
-// flight).  The grid is timed at 4, 8, 16 and 32 workgroups per CU and the fastest is
-// reported: the ceiling is the best this hardware does with the pattern.  Kinds:
-//   read   - sum of every float of a (one float per thread written back, so the loads stay);
-//   write  - fill b;
-//   copy   - b = a through registers;
+// Every probe streams whole arrays with 16-byte accesses per lane, the loads and stores
+// non-temporal (streamed data is not kept in the caches), and the grid is timed at 2, 4, 8, 16, 32
+// and 64 workgroups per CU with the fastest reported: the ceiling is the best this hardware does
+// with the pattern.  (Cached accesses and at least 4 workgroups per CU measured 12-20 % lower.)
+// Kinds:
+//   read   - sum of every float of a, grid-stride, eight requests in flight per lane (one float per
+//            thread written back, so the loads stay);
+//   write  - fill b, grid-stride;
+//   copy   - b = a through registers, each workgroup copying contiguous runs of 8 x 256 float4s;
 //   dma    - b = a through LDS: global_load_lds_dwordx4 stages a wave's 1 KiB, the wave
 //            waits for it, reads it back and stores it (the staging the step kernels use);
 //   mix10_3 / mix12_6 - out_m = sum_k c_mk in_k over 10 inputs and 3 outputs (the velocity
@@ -26,49 +28,58 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kUnroll = 4;
+constexpr int kReadUnroll = 8;
+constexpr int kCopyRun = 8;
 
-__global__ __launch_bounds__(kBlock) void read_probe(const float4* __restrict__ a, std::int64_t n4, float* out) {
+// Clang's native four-float vector (the non-temporal builtins take vectors of scalars).
+typedef float v4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4 load_nt(const v4* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void store_nt(v4 v, v4* p) { __builtin_nontemporal_store(v, p); }
+
+__global__ __launch_bounds__(kBlock) void read_probe(const v4* __restrict__ a, std::int64_t n4, float* out) {
     const std::int64_t stride = std::int64_t(gridDim.x) * kBlock;
     std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    float4 acc{0, 0, 0, 0};
-    for (; i + (kUnroll - 1) * stride < n4; i += kUnroll * stride) {
-        float4 v[kUnroll];
+    v4 acc = {0, 0, 0, 0};
+    for (; i + (kReadUnroll - 1) * stride < n4; i += kReadUnroll * stride) {
+        v4 v[kReadUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) v[u] = a[i + u * stride];
+        for (int u = 0; u < kReadUnroll; ++u) v[u] = load_nt(a + i + u * stride);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) { acc.x += v[u].x; acc.y += v[u].y; acc.z += v[u].z; acc.w += v[u].w; }
+        for (int u = 0; u < kReadUnroll; ++u) acc += v[u];
     }
-    for (; i < n4; i += stride) {
-        const float4 v = a[i];
-        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-    }
+    for (; i < n4; i += stride) acc += load_nt(a + i);
     out[std::int64_t(blockIdx.x) * kBlock + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
 }
 
-__global__ __launch_bounds__(kBlock) void write_probe(float4* __restrict__ b, std::int64_t n4, float value) {
+__global__ __launch_bounds__(kBlock) void write_probe(v4* __restrict__ b, std::int64_t n4, float value) {
     const std::int64_t stride = std::int64_t(gridDim.x) * kBlock;
-    const float4 v{value, value, value, value};
-    for (std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n4; i += stride) b[i] = v;
+    const v4 v = {value, value, value, value};
+    for (std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n4; i += stride) store_nt(v, b + i);
 }
 
-__global__ __launch_bounds__(kBlock) void copy_probe(const float4* __restrict__ a, float4* __restrict__ b,
-                                                     std::int64_t n4) {
-    const std::int64_t stride = std::int64_t(gridDim.x) * kBlock;
-    std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    for (; i + (kUnroll - 1) * stride < n4; i += kUnroll * stride) {
-        float4 v[kUnroll];
+// Each workgroup copies runs of kCopyRun * kBlock consecutive float4s: all of a run's loads are in
+// flight before its stores.
+__global__ __launch_bounds__(kBlock) void copy_probe(const v4* __restrict__ a, v4* __restrict__ b, std::int64_t n4) {
+    constexpr std::int64_t run = std::int64_t(kCopyRun) * kBlock;
+    for (std::int64_t base = std::int64_t(blockIdx.x) * run; base < n4; base += std::int64_t(gridDim.x) * run) {
+        v4 v[kCopyRun];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) v[u] = a[i + u * stride];
+        for (int u = 0; u < kCopyRun; ++u) {
+            const std::int64_t i = base + u * kBlock + threadIdx.x;
+            v[u] = i < n4 ? load_nt(a + i) : v4{0, 0, 0, 0};
+        }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) b[i + u * stride] = v[u];
+        for (int u = 0; u < kCopyRun; ++u) {
+            const std::int64_t i = base + u * kBlock + threadIdx.x;
+            if (i < n4) store_nt(v[u], b + i);
+        }
     }
-    for (; i < n4; i += stride) b[i] = a[i];
 }
 
 // Each wave owns a 1 KiB LDS slot per unrolled request.
-__global__ __launch_bounds__(kBlock) void dma_probe(const float4* __restrict__ a, float4* __restrict__ b,
-                                                    std::int64_t n4) {
-    __shared__ float4 slot[kUnroll][kBlock];
+__global__ __launch_bounds__(kBlock) void dma_probe(const v4* __restrict__ a, v4* __restrict__ b, std::int64_t n4) {
+    __shared__ v4 slot[kUnroll][kBlock];
     const std::int64_t stride = std::int64_t(gridDim.x) * kBlock;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -82,35 +93,32 @@ __global__ __launch_bounds__(kBlock) void dma_probe(const float4* __restrict__ a
         __builtin_amdgcn_s_waitcnt(0x0070 | 0x0f00);   // vmcnt(0)
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) b[i + u * stride] = slot[u][wave * 64 + lane];
+        for (int u = 0; u < kUnroll; ++u) store_nt(slot[u][wave * 64 + lane], b + i + u * stride);
         __builtin_amdgcn_wave_barrier();
     }
-    for (; i < n4; i += stride) b[i] = a[i];
+    for (; i < n4; i += stride) store_nt(load_nt(a + i), b + i);
 }
 
 template <int IN, int OUT>
 __global__ __launch_bounds__(kBlock) void mix_probe(const float* const* __restrict__ in, float* const* __restrict__ out,
                                                    std::int64_t n4) {
     const std::int64_t stride = std::int64_t(gridDim.x) * kBlock;
-    const float4* src[IN];
-    float4* dst[OUT];
+    const v4* src[IN];
+    v4* dst[OUT];
 #pragma unroll
-    for (int k = 0; k < IN; ++k) src[k] = reinterpret_cast<const float4*>(in[k]);
+    for (int k = 0; k < IN; ++k) src[k] = reinterpret_cast<const v4*>(in[k]);
 #pragma unroll
-    for (int m = 0; m < OUT; ++m) dst[m] = reinterpret_cast<float4*>(out[m]);
+    for (int m = 0; m < OUT; ++m) dst[m] = reinterpret_cast<v4*>(out[m]);
     for (std::int64_t i = std::int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n4; i += stride) {
-        float4 v[IN];
+        v4 v[IN];
 #pragma unroll
-        for (int k = 0; k < IN; ++k) v[k] = src[k][i];
+        for (int k = 0; k < IN; ++k) v[k] = load_nt(src[k] + i);
 #pragma unroll
         for (int m = 0; m < OUT; ++m) {
-            float4 r{0, 0, 0, 0};
+            v4 r = {0, 0, 0, 0};
 #pragma unroll
-            for (int k = 0; k < IN; ++k) {
-                const float c = 1.0f / float(1 + k + m);
-                r.x += c * v[k].x; r.y += c * v[k].y; r.z += c * v[k].z; r.w += c * v[k].w;
-            }
-            dst[m][i] = r;
+            for (int k = 0; k < IN; ++k) r += (1.0f / float(1 + k + m)) * v[k];
+            store_nt(r, dst[m] + i);
         }
     }
 }
@@ -136,7 +144,7 @@ extern "C" int seis3d_bandwidth_probe(int kind, uint64_t bytes, int reps, double
         int dev = 0, cus = 0;
         check(hipGetDevice(&dev), "query device");
         check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "query device");
-        const int max_grid = std::max(1, cus) * 32;
+        const int max_grid = std::max(1, cus) * 64;
         std::vector<float*> arrays(std::size_t(streams_in + streams_out), nullptr);
         float* partial = nullptr;
         float** table = nullptr;
@@ -164,8 +172,8 @@ extern "C" int seis3d_bandwidth_probe(int kind, uint64_t bytes, int reps, double
             check(hipMemcpyAsync(table, arrays.data(), arrays.size() * sizeof(float*), hipMemcpyHostToDevice, s),
                   "upload probe table");
             auto launch = [&](int grid) {
-                const auto* a = reinterpret_cast<const float4*>(arrays[0]);
-                auto* b = reinterpret_cast<float4*>(arrays.back());
+                const auto* a = reinterpret_cast<const v4*>(arrays[0]);
+                auto* b = reinterpret_cast<v4*>(arrays.back());
                 switch (kind) {
                 case 0: hipLaunchKernelGGL(read_probe, dim3(grid), dim3(kBlock), 0, s, a, n4, partial); break;
                 case 1: hipLaunchKernelGGL(write_probe, dim3(grid), dim3(kBlock), 0, s, b, n4, 1.0f); break;
@@ -181,7 +189,7 @@ extern "C" int seis3d_bandwidth_probe(int kind, uint64_t bytes, int reps, double
                 check(hipGetLastError(), "kernel launch");
             };
             double best = 0;
-            for (int per_cu : {4, 8, 16, 32}) {
+            for (int per_cu : {2, 4, 8, 16, 32, 64}) {
                 const int grid = std::max(1, cus) * per_cu;
                 launch(grid);   // warm-up
                 check(hipEventRecord(e0, s), "record event");
