@@ -1372,9 +1372,8 @@ void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_step
     bool any_free = false;
     for (int s = 0; s < S; ++s) {
         seis3d_t* t = b->sessions[s];
-        // 64x4 tiles of 16 planes for every shot: items of equal work, so that equal shares of
-        // the item list are equal shares of the work for the eight XCDs.
-        maps[s] = seis3d::dma_tile_map(t->base().g, 4, std::min(seis3d::kQueueChunk, std::max(1, t->c.gz - 8)));
+        // 64x4 tiles for every shot, with the shot's own (autotuned) z-chunk length.
+        maps[s] = seis3d::dma_tile_map(t->base().g, 4, multi_shot_shape(t).z_chunk);
         first[std::size_t(s) + 1] = first[std::size_t(s)] + maps[s].nblocks;
         spacing[s] = t->c.spacing;
         max_gx = std::max(max_gx, t->c.gx);
@@ -1383,6 +1382,29 @@ void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_step
         any_free = any_free || t->c.free_surface;
     }
     const int total = first[std::size_t(S)];
+    // The eight XCDs' ranges of the item list, of equal work: an item of shot s marches its
+    // chunk's planes plus 8 halo planes (counted as half a plane each, the halo planes stage only
+    // the tile's own rows).
+    std::vector<int> xcd_first(9, 0);
+    {
+        std::vector<double> work(static_cast<std::size_t>(S));
+        double all = 0;
+        for (int s = 0; s < S; ++s) {
+            const double per_item = double(b->sessions[s]->c.gz - 8) / maps[s].nbz + 4.0;
+            work[std::size_t(s)] = per_item;
+            all += per_item * maps[s].nblocks;
+        }
+        int x = 1;
+        double acc = 0;
+        for (int s = 0; s < S && x < 8; ++s) {
+            for (int k = 0; k < maps[s].nblocks && x < 8; ++k) {
+                acc += work[std::size_t(s)];
+                while (x < 8 && acc >= all * x / 8) xcd_first[std::size_t(x++)] = first[std::size_t(s)] + k + 1;
+            }
+        }
+        while (x < 8) xcd_first[std::size_t(x++)] = total;
+        xcd_first[8] = total;
+    }
     for (int64_t n = 0; n < n_steps; ++n) {
         for (int s = 0; s < S; ++s) {
             seis3d_t* t = b->sessions[s];
@@ -1413,7 +1435,7 @@ void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_step
     check(hipMalloc(&b->d_steps, table.size() * sizeof(seis3d::DevStep)), "allocate batch tables");
     check(hipMalloc(&b->d_spacing, spacing.size() * sizeof(float)), "allocate batch tables");
     check(hipMalloc(&b->d_maps, maps.size() * sizeof(seis3d::TileMap)), "allocate batch tables");
-    check(hipMalloc(&b->d_first, first.size() * sizeof(int)), "allocate batch tables");
+    check(hipMalloc(&b->d_first, (first.size() + xcd_first.size()) * sizeof(int)), "allocate batch tables");
     check(hipMalloc(&b->d_counters, ncounters * sizeof(unsigned)), "allocate batch tables");
     if (!samples.empty())
         check(hipMalloc(&b->d_samples, samples.size() * sizeof(seis3d::ShotSample)), "allocate batch tables");
@@ -1425,6 +1447,10 @@ void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_step
     check(hipMemcpy(b->d_maps, maps.data(), maps.size() * sizeof(seis3d::TileMap), hipMemcpyHostToDevice),
           "upload batch tables");
     check(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice),
+          "upload batch tables");
+    const int* d_xcd_first = b->d_first + first.size();
+    check(hipMemcpy(b->d_first + first.size(), xcd_first.data(), xcd_first.size() * sizeof(int),
+                    hipMemcpyHostToDevice),
           "upload batch tables");
     if (!samples.empty())
         check(hipMemcpy(b->d_samples, samples.data(), samples.size() * sizeof(seis3d::ShotSample),
@@ -1439,10 +1465,11 @@ void capture_queue(seis3d_batch* b, const seis3d_source* sources, int64_t n_step
     for (int64_t n = 0; n < n_steps; ++n) {
         const seis3d::DevStep* vel = b->d_steps + std::size_t(2 * n) * S;
         unsigned* q = b->d_counters + std::size_t(n) * 16;
-        check(seis3d::launch_dma_queue(true, vel, b->d_maps, b->d_first, q, S, total, blocks, st), "kernel launch");
+        check(seis3d::launch_dma_queue(true, vel, b->d_maps, b->d_first, d_xcd_first, q, S, blocks, st),
+              "kernel launch");
         if (any_free)
             check(seis3d::launch_velocity_ghost_shots(vel, b->d_spacing, S, max_gx, max_gy, st), "kernel launch");
-        check(seis3d::launch_dma_queue(false, vel + S, b->d_maps, b->d_first, q + 8, S, total, blocks, st),
+        check(seis3d::launch_dma_queue(false, vel + S, b->d_maps, b->d_first, d_xcd_first, q + 8, S, blocks, st),
               "kernel launch");
         const std::size_t k = start[std::size_t(n) + 1] - start[std::size_t(n)];
         if (k)

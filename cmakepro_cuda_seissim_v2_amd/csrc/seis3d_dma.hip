@@ -142,7 +142,7 @@ struct VelOperands {
 };
 
 constexpr int kMaxChunk = 128;   // planes per tile (z table capacity)
-static_assert(kQueueChunk <= kMaxChunk, "work-queue z-chunks fit the z table");
+
 
 // Copy the tile's slices of the three CPML tables ([half][n][a,b,1/kappa])
 // into LDS as [half*3 + k][index]; entries outside a slab are never read.
@@ -770,17 +770,18 @@ __global__ __launch_bounds__(64 * BY) void stress_dma_shots(ShotLaunch L) {
 
 // Work-queue half-steps: a persistent grid (every workgroup resident) pulls (shot, tile, z-chunk)
 // items from per-XCD queues until they run dry.  The items of all shots form one list (shot s owns
-// items [first[s], first[s + 1])), dealt to the eight XCDs in contiguous eighths so that
-// neighbouring tiles share an XCD's L2, as tile_of deals them; within its eighth an XCD's
-// workgroups take the next item from the XCD's counter, so shots of uneven size balance.
-// counters[0..7] must be zero at launch (one set per launch).  Every workgroup leaves its loop
-// as soon as its XCD's counter passes the end of the XCD's range, so the grid always drains.
+// items [first[s], first[s + 1])), dealt to the eight XCDs in contiguous ranges of equal work
+// (XCD x owns items [xcd_first[x], xcd_first[x + 1])) so that neighbouring tiles share an XCD's
+// L2, as tile_of deals them; within its range an XCD's workgroups take the next item from the
+// XCD's counter, so shots of uneven size balance.  counters[0..7] must be zero at launch (one set
+// per launch).  Every workgroup leaves its loop as soon as its XCD's counter passes the end of
+// the XCD's range, so the grid always drains.
 __device__ __forceinline__ bool queue_item(const TileMap* __restrict__ maps, const int* __restrict__ first,
-                                           unsigned* counters, int nshots, int total, int* slot, int& s, int& bx,
-                                           int& by, int& bz) {
+                                           const int* __restrict__ xcd_first, unsigned* counters, int nshots,
+                                           int* slot, int& s, int& bx, int& by, int& bz) {
     const int xcd = int(blockIdx.x) & 7;
-    const int lo = int((static_cast<long long>(xcd) * total) / 8);
-    const int hi = int((static_cast<long long>(xcd + 1) * total) / 8);
+    const int lo = xcd_first[xcd];
+    const int hi = xcd_first[xcd + 1];
     __syncthreads();   // every wave has finished with the previous item (its LDS and *slot)
     if (threadIdx.x == 0 && threadIdx.y == 0) *slot = lo + int(atomicAdd(&counters[xcd], 1u));
     __syncthreads();
@@ -806,23 +807,24 @@ using ConstStep = const __attribute__((address_space(4))) DevStep;
 template <int BY>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma_queue(
     const DevStep* __restrict__ steps, const TileMap* __restrict__ maps, const int* __restrict__ first,
-    unsigned* counters, int nshots, int total) {
+    const int* __restrict__ xcd_first, unsigned* counters, int nshots) {
     __shared__ VelShared<BY> sh;
     __shared__ int slot;
     int s, bx, by, bz;
-    while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
+    while (queue_item(maps, first, xcd_first, counters, nshots, &slot, s, bx, by, bz))
         velocity_tile<BY, false, true>(sh, ((ConstStep*)steps)[s], maps[s], bx, by, bz);
 }
 
 template <int BY>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void stress_dma_queue(const DevStep* __restrict__ steps,
                                                           const TileMap* __restrict__ maps,
-                                                          const int* __restrict__ first, unsigned* counters,
-                                                          int nshots, int total) {
+                                                          const int* __restrict__ first,
+                                                          const int* __restrict__ xcd_first, unsigned* counters,
+                                                          int nshots) {
     __shared__ StrShared<BY> sh;
     __shared__ int slot;
     int s, bx, by, bz;
-    while (queue_item(maps, first, counters, nshots, total, &slot, s, bx, by, bz))
+    while (queue_item(maps, first, xcd_first, counters, nshots, &slot, s, bx, by, bz))
         stress_tile<BY, false, true>(sh, ((ConstStep*)steps)[s], maps[s], bx, by, bz);
 }
 
@@ -851,14 +853,14 @@ hipError_t queue_grid(int* blocks) {
 }
 
 hipError_t launch_dma_queue(bool velocity, const DevStep* steps, const TileMap* maps, const int* first,
-                            unsigned* counters, int nshots, int total, int blocks, hipStream_t s) {
-    if (nshots < 1 || total < 1 || blocks < 8 || blocks % 8) return hipErrorInvalidValue;
+                            const int* xcd_first, unsigned* counters, int nshots, int blocks, hipStream_t s) {
+    if (nshots < 1 || blocks < 8 || blocks % 8) return hipErrorInvalidValue;
     if (velocity)
         hipLaunchKernelGGL(velocity_dma_queue<4>, dim3(unsigned(blocks)), dim3(64, 4), 0, s, steps, maps, first,
-                           counters, nshots, total);
+                           xcd_first, counters, nshots);
     else
         hipLaunchKernelGGL(stress_dma_queue<4>, dim3(unsigned(blocks)), dim3(64, 4), 0, s, steps, maps, first,
-                           counters, nshots, total);
+                           xcd_first, counters, nshots);
     return hipGetLastError();
 }
 

@@ -53,14 +53,13 @@ TileMap dma_tile_map(const DevGrid& g, int tile_y, int zchunk);
 // One LDS-DMA half-step (in place) of the L.nshots shots of L, one launch; all share tile_y.
 hipError_t launch_dma_shots(bool velocity, const ShotLaunch& L, int tile_y, hipStream_t s);
 // Work-queue half-step (64x4 tiles, in place) over the tiles of nshots shots: steps and maps are
-// device arrays of nshots, first holds nshots + 1 item offsets (first[nshots] == total), counters
-// eight zeroed words; `blocks` is a whole number of rounds of eight (queue_grid: every workgroup
-// resident).
+// device arrays of nshots, first holds nshots + 1 item offsets, xcd_first the 9 item offsets of
+// the eight XCDs' ranges (xcd_first[8] == first[nshots]), counters eight zeroed words; `blocks` is
+// a whole number of rounds of eight (queue_grid: every workgroup resident).  Each shot's z-chunk
+// length (its TileMap's) is at most 128 planes.
 hipError_t launch_dma_queue(bool velocity, const DevStep* steps, const TileMap* maps, const int* first,
-                            unsigned* counters, int nshots, int total, int blocks, hipStream_t s);
+                            const int* xcd_first, unsigned* counters, int nshots, int blocks, hipStream_t s);
 hipError_t queue_grid(int* blocks);
-// Planes per work-queue item.
-constexpr int kQueueChunk = 16;
 // Zero n words (graph-safe: a kernel, not a memset node).
 hipError_t launch_zero_words(unsigned* w, long long n, hipStream_t s);
 // Resident workgroups of the family-4 pair per CU (the smaller of velocity and

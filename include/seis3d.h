@@ -293,7 +293,8 @@ int seis3d_batch_create(seis3d_t* const* sessions, const seis3d_source* sources,
  * launch per half-step for all its shots (the shots' tiles dealt out as one list) on its own
  * stream.  WORK_QUEUE: one persistent launch per half-step for every shot of the batch: a grid of
  * resident 64x4 workgroups pulls (shot, tile, z-chunk) items from per-XCD queues, so shots of
- * uneven size share the GPU without a tail per shot; each shot is stepped in 64x4x16 items.
+ * uneven size share the GPU without a tail per shot; each shot is stepped in 64x4 tiles with its own
+ * (autotuned) z-chunk length.
  * Shots with reflecting walls or a learned frame make the last two fall back to SHOT_STREAMS.
  * Every mode gives every shot's gather bit for bit as the shot run alone. */
 enum seis3d_batch_mode {
