@@ -28,6 +28,7 @@ pytestmark = pytest.mark.gpu
 
 TOL32 = 1e-5
 TOL64 = 1e-4
+ORACLE_CHUNK = 200   # oracle steps between progress lines
 
 
 def rel_l2(a, b):
@@ -36,8 +37,26 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+_capture = None
+
+
+@pytest.fixture(autouse=True)
+def _progress(request):
+    """Hands log() pytest's capture manager for the length of one test."""
+    global _capture
+    _capture = request.config.pluginmanager.getplugin("capturemanager")
+    yield
+    _capture = None
+
+
 def log(msg):
-    print(msg, flush=True)
+    """A progress line that goes past pytest's output capture, so that the minutes-long oracle
+    runs below print as they go, also under -q."""
+    if _capture is not None:
+        with _capture.global_and_fixture_disabled():
+            print(msg, flush=True)
+    else:
+        print(msg, flush=True)
     sys.stdout.flush()
 
 
@@ -60,7 +79,12 @@ def oracle_parity(ctx, model, src, rec, steps, stride, gather, g, name):
         with O.Oracle3D(ctx, precision) as o:
             o.upload_model(*model)
             o.set_receivers(rec)
-            ref = o.run(src, 0, steps, stride=stride, max_samples=steps // stride)
+            parts = []
+            for first in range(0, steps, ORACLE_CHUNK):
+                n = min(ORACLE_CHUNK, steps - first)
+                parts.append(o.run(src, first, n, stride=stride, max_samples=n // stride + 1))
+                log(f"{name}: FP{precision} oracle at step {first + n}/{steps}")
+            ref = np.concatenate(parts)
             e = rel_l2(gather, ref)
             fe = max(rel_l2(gpu_fields[f], o.field(f)[core]) for f in range(9))
         log(f"{name}: gather rel-L2 {e:.3e}, worst field rel-L2 {fe:.3e} vs FP{precision} (bar {tol:g})")
