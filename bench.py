@@ -416,10 +416,11 @@ def bench_2d(args) -> None:
 
 
 def bench_abc(args) -> None:
-    """Learned 8-cell absorbing boundary on 300 x 300 x 400 (36 M nodes), 1500 steps, free top,
-    layered-plus-random-field model: reflected energy against a 40-cell CPML run (the reflection-free reference) for the
-    learned frame (weights shipped in the package), the frame without its network, plain 8-cell CPML
-    and plain 20-cell CPML (the fallback); and the time per step of each boundary."""
+    """Learned 8-cell absorbing boundaries on 300 x 300 x 400 (36 M nodes), 1500 steps, free top,
+    layered-plus-random-field model: reflected energy against a 40-cell CPML run (the reflection-free
+    reference) for the learned frame (MLP weights shipped in the package), the frame without its
+    network, plain 8-cell CPML, 8-cell CPML with the learned profile (shipped in the package) and plain
+    20-cell CPML (the fallback); and the time per step of each boundary."""
     from cmakepro_cuda_seissim_v2_amd import learned_abc as LA
     from cmakepro_cuda_seissim_v2_amd import seis3d as S
     from cmakepro_cuda_seissim_v2_amd import workloads as W
@@ -431,11 +432,14 @@ def bench_abc(args) -> None:
     src = S.default_source(x=nx // 2, y=ny // 2, z=max(2, nz // 4), kind=S.EXPLOSION, amplitude=1e8)
     vmax = float(model[0].max())
     times = {}
-    for case, pml in (("learned", 8), ("cpml8", 8), ("cpml20", 20), ("cpml40", 40)):
+    profile = LA.load_profile()
+    for case, pml in (("learned", 8), ("cpml8", 8), ("profile", 8), ("cpml20", 20), ("cpml40", 40)):
         ctx = W.imported_context(nx, ny, nz, max_vp=vmax, pml=pml, free_surface=True, frequency=25.0)
         g = LA.session(ctx, model)
         if case == "learned":
             g.abc_attach(params)
+        elif case == "profile":
+            g.set_cpml_profile(profile)
         g.run(src, 0, 20)
         g.synchronize()
         t0 = time.perf_counter()
@@ -445,8 +449,8 @@ def bench_abc(args) -> None:
         g.release()
         print(f"bench: abc {case} {times[case]:.3f} ms/step", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    e = LA.reflection(params, model, src, steps, every=100,
-                      log=lambda m: print(f"bench: abc {m}", file=sys.stderr, flush=True))
+    e = LA.reflection(params, model, src, steps, every=100, cases=("learned", "none", "cpml8", "profile", "cpml20"),
+                      log=lambda m: print(f"bench: abc {m}", file=sys.stderr, flush=True), profile=profile)
     secs = time.perf_counter() - t0
     print(f"bench: abc reflected energy {e} ({secs:.1f} s)", file=sys.stderr, flush=True)
     out = {
@@ -461,6 +465,8 @@ def bench_abc(args) -> None:
             "workload": f"3D O(8), {nx}x{ny}x{nz}, dx=10 m, free top, {steps} steps, sampled every 100 steps",
             "reflected_energy": e,
             "gain_vs_frame_without_network": e["none"] / e["learned"],
+            "learned_profile": list(profile),
+            "learned_profile_gain_vs_cpml8": e["cpml8"] / e["profile"],
             "ms_per_step": {k: round(v, 4) for k, v in times.items()},
         },
     }
@@ -681,6 +687,7 @@ def main():
         out["grid_200"] = grid_200(args, S, W)
         out["fwi"] = fwi_leg(S, W)
         out["batch"] = batch_run(800)
+        out["learned_boundary"] = abc_leg()
     print(json.dumps(out), flush=True)
 
 
@@ -756,6 +763,25 @@ def grid_200(args, S, W) -> dict:
     }
     print(f"bench: grid_200 {out}", file=sys.stderr, flush=True)
     return out
+
+
+def abc_leg() -> dict:
+    """The learned boundaries on cases outside their training sets (64^3, free top, 360 steps, random
+    source kind, position and 12-30 Hz frequency): reflected energy against a 40-cell CPML run for the
+    reference's 8-cell CPML profile and the shipped learned profile (same step kernels, same cost)."""
+    from cmakepro_cuda_seissim_v2_amd import learned_abc as LA
+    profile = LA.load_profile()
+    rows = []
+    t0 = time.perf_counter()
+    for seed in LA.PROFILE_HELD_OUT_SEEDS[:2]:
+        case = LA.profile_case(seed)
+        ref, got = LA.profile_reflection(None, case), LA.profile_reflection(profile, case)
+        rows.append({"seed": seed, "frequency_hz": round(case["frequency"], 2), "cpml8_reference_profile": float(f"{ref:.4e}"),
+                     "cpml8_learned_profile": float(f"{got:.4e}"), "gain": round(ref / got, 2)})
+        print(f"bench: learned profile held-out {rows[-1]}", file=sys.stderr, flush=True)
+    return {"workload": "8-cell CPML, learned profile vs the reference profile, 64^3 held-out cases, 360 steps, "
+                        "relative squared velocity error vs 40-cell CPML, sampled every 8 steps",
+            "learned_profile": list(profile), "held_out": rows, "seconds": round(time.perf_counter() - t0, 1)}
 
 
 def fwi_leg(S, W) -> dict:

@@ -82,7 +82,7 @@ void enumerate(const seis3d_ctx& c, int px, std::vector<int>& cells, std::vector
 
 void upload_tables(seis3d_t* t, bool identity) {
     for (int a = 0; a < 3; ++a) {
-        std::vector<float> v = make_coefficients(t->c, a);
+        std::vector<float> v = make_coefficients(t->c, a, t->cpml_profile());
         if (identity)
             for (std::size_t k = 0; k < v.size(); k += 3) { v[k] = 0.0f; v[k + 1] = 0.0f; v[k + 2] = 1.0f; }
         check(hipMemcpyAsync(t->coef[a], v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, t->stream),

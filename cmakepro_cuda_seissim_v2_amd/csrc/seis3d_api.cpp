@@ -110,20 +110,32 @@ ViewLayout make_view_layout(const seis3d_ctx& c) {
     return l;
 }
 
-// makeCoefficients3D (cuda/Cuda_Seis3D.cu:45-64).
-std::vector<float> make_coefficients(const seis3d_ctx& c, int axis) {
+// makeCoefficients3D (cuda/Cuda_Seis3D.cu:45-64); with a learned profile, the same layer depth,
+// a/b/kappa formulas and table layout with the shapes of seis3d_set_cpml_profile.
+std::vector<float> make_coefficients(const seis3d_ctx& c, int axis, const double* profile) {
     const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
     std::vector<float> values(std::size_t(n) * 6);
-    const double sigma_max = -4.0 * c.max_vp * std::log(1.0e-6) / (2.0 * c.pml * c.spacing);
+    double sigma_max = -4.0 * c.max_vp * std::log(1.0e-6) / (2.0 * c.pml * c.spacing);
+    if (profile) sigma_max *= std::exp(profile[0]);
     for (int half = 0; half < 2; ++half)
         for (int i = 0; i < n; ++i) {
             const double coordinate = i + 0.5 * half;
             double depth = (coordinate - (n - kRadius - c.pml - 0.5)) / c.pml;
             if (!(axis == 2 && c.free_surface)) depth = std::max(depth, (kRadius + c.pml - 0.5 - coordinate) / c.pml);
             depth = std::clamp(depth, 0.0, 1.0);
-            const double power = depth * depth * depth;
-            const double sigma = sigma_max * power, kappa = 1.0 + 2.0 * power;
-            const double alpha = depth > 0 ? 3.141592653589793 * c.frequency * (1.0 - depth) : 0.0;
+            double power, sigma, kappa, alpha;
+            if (!profile) {
+                power = depth * depth * depth;
+                sigma = sigma_max * power;
+                kappa = 1.0 + 2.0 * power;
+                alpha = depth > 0 ? 3.141592653589793 * c.frequency * (1.0 - depth) : 0.0;
+            } else {
+                power = std::pow(depth, profile[1]);
+                sigma = sigma_max * power;
+                kappa = 1.0 + (profile[2] - 1.0) * power;
+                alpha = depth > 0 ? profile[3] * 3.141592653589793 * c.frequency * std::pow(1.0 - depth, profile[4])
+                                  : 0.0;
+            }
             const double b = std::exp(-(sigma / kappa + alpha) * c.dt);
             const double a = sigma > 0 ? sigma * (b - 1.0) / (kappa * (sigma + kappa * alpha)) : 0.0;
             const std::size_t o = (std::size_t(half) * n + i) * 3;
@@ -132,6 +144,15 @@ std::vector<float> make_coefficients(const seis3d_ctx& c, int axis) {
             values[o + 2] = float(1.0 / kappa);
         }
     return values;
+}
+
+void check_profile(const double* p) {
+    if (!p) invalid("Seis3D: null CPML profile.");
+    for (int k = 0; k < SEIS3D_PROFILE_PARAMS; ++k)
+        if (!std::isfinite(p[k])) invalid("Seis3D: CPML profile parameters must be finite.");
+    if (std::fabs(p[0]) > 5 || p[1] < 1 || p[1] > 8 || p[2] < 1 || p[2] > 20 || p[3] < 0 || p[3] > 10 ||
+        p[4] < 0.1 || p[4] > 8)
+        invalid("Seis3D: CPML profile parameters out of range.");
 }
 
 // Ricker sample of sourceKernel3D (cuda/Cuda_Seis3D.cu:319-320): FP64 phase,
@@ -190,7 +211,7 @@ void alloc_grid(float*& p, std::size_t cells, hipStream_t s, const char* what) {
 
 void upload_coefficients(seis3d_t* t, const seis3d_ctx& c, float* const dst[3]) {
     for (int a = 0; a < 3; ++a) {
-        const auto v = make_coefficients(c, a);
+        const auto v = make_coefficients(c, a, t->cpml_profile());
         check(hipMemcpyAsync(dst[a], v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, t->stream),
               "upload CPML coefficients");
     }
@@ -607,6 +628,47 @@ int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx) {
     });
 }
 
+int seis3d_set_cpml_profile(seis3d_t* t, const double* profile) {
+    return guarded([&] {
+        require_unbatched(t, "a CPML profile change");
+        if (profile) check_profile(profile);
+        if (t->abc) fail(SEIS3D_ELIFECYCLE, "Seis3D: detach the learned boundary before changing the CPML profile.");
+        const bool had = t->has_profile;
+        double old[SEIS3D_PROFILE_PARAMS];
+        std::copy(t->profile, t->profile + SEIS3D_PROFILE_PARAMS, old);
+        t->has_profile = profile != nullptr;
+        if (profile) std::copy(profile, profile + SEIS3D_PROFILE_PARAMS, t->profile);
+        if (t->c.reflecting) return;
+        try {
+            upload_coefficients(t, t->c, t->coef);
+        } catch (...) {
+            t->has_profile = had;
+            std::copy(old, old + SEIS3D_PROFILE_PARAMS, t->profile);
+            throw;
+        }
+    });
+}
+
+int seis3d_get_cpml_profile(const seis3d_t* t, double* profile, int* learned) {
+    return guarded([&] {
+        require_session(t);
+        static const double reference[SEIS3D_PROFILE_PARAMS] = {0.0, 3.0, 3.0, 1.0, 1.0};
+        if (profile) std::copy_n(t->has_profile ? t->profile : reference, SEIS3D_PROFILE_PARAMS, profile);
+        if (learned) *learned = t->has_profile ? 1 : 0;
+    });
+}
+
+int seis3d_cpml_profile_coefficients(const seis3d_ctx* ctx, const double* profile, int axis, float* out,
+                                     size_t capacity) {
+    return guarded([&] {
+        if (!ctx || !out || axis < 0 || axis > 2) invalid("Seis3D: invalid coefficient request.");
+        if (profile) check_profile(profile);
+        const auto v = make_coefficients(*ctx, axis, profile);
+        if (capacity < v.size()) invalid("Seis3D: coefficient buffer too small.");
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
 int seis3d_read_cpml(seis3d_t* t, int axis, float* out, size_t capacity) {
     return guarded([&] {
         require_session(t);
@@ -849,15 +911,28 @@ int seis3d_gather_end(seis3d_t* t) {
     });
 }
 
+namespace {
+// The session's packed transfer buffer (one unpadded-pitch field), allocated on first use.
+void ensure_stage(seis3d_t* t) {
+    if (t->stage) return;
+    check(hipMalloc(&t->stage, std::size_t(t->c.gx) * t->c.gy * t->c.gz * sizeof(float)), "allocate transfer buffer");
+}
+}  // namespace
+
 int seis3d_download_field(seis3d_t* t, int field, float* out) {
     return guarded([&] {
         require_session(t);
         if (field < 0 || field > 11 || !out) invalid("Seis3D: invalid field request.");
         const float* src = field < 9 ? t->fields[field] : field == 9 ? t->inv_rho : field == 10 ? t->lambda : t->mu;
         const seis3d_ctx& c = t->c;
-        check(hipMemcpy2DAsync(out, std::size_t(c.gx) * 4, src, std::size_t(t->plan.px) * 4, std::size_t(c.gx) * 4,
-                               std::size_t(c.gy) * c.gz, hipMemcpyDeviceToHost, t->stream),
+        // Pitched rows are packed on the device, then cross PCIe in one contiguous copy (a pitched
+        // copy straight into pageable host memory moves row by row).
+        const std::size_t rows = std::size_t(c.gy) * c.gz;
+        ensure_stage(t);
+        check(hipMemcpy2DAsync(t->stage, std::size_t(c.gx) * 4, src, std::size_t(t->plan.px) * 4,
+                               std::size_t(c.gx) * 4, rows, hipMemcpyDeviceToDevice, t->stream),
               "download field");
+        check(hipMemcpyAsync(out, t->stage, rows * c.gx * 4, hipMemcpyDeviceToHost, t->stream), "download field");
         check(hipStreamSynchronize(t->stream), "finish field download");
     });
 }
@@ -867,8 +942,11 @@ int seis3d_upload_field(seis3d_t* t, int field, const float* in) {
         require_session(t);
         if (field < 0 || field > 8 || !in) invalid("Seis3D: invalid field request.");
         const seis3d_ctx& c = t->c;
-        check(hipMemcpy2DAsync(t->fields[field], std::size_t(t->plan.px) * 4, in, std::size_t(c.gx) * 4,
-                               std::size_t(c.gx) * 4, std::size_t(c.gy) * c.gz, hipMemcpyHostToDevice, t->stream),
+        const std::size_t rows = std::size_t(c.gy) * c.gz;
+        ensure_stage(t);
+        check(hipMemcpyAsync(t->stage, in, rows * c.gx * 4, hipMemcpyHostToDevice, t->stream), "upload field");
+        check(hipMemcpy2DAsync(t->fields[field], std::size_t(t->plan.px) * 4, t->stage, std::size_t(c.gx) * 4,
+                               std::size_t(c.gx) * 4, rows, hipMemcpyDeviceToDevice, t->stream),
               "upload field");
         check(hipStreamSynchronize(t->stream), "finish field upload");
         t->ghosts_valid = false;

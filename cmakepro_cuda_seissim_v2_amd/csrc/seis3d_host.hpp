@@ -82,7 +82,10 @@ void validate(seis3d_ctx& c);
 // One material cell: finite, 0 < Vp <= 30000, 0 <= Vs < Vp, positive bulk modulus, 0 < rho <= 50000.
 bool valid_material(float vp, float vs, float rho);
 Plan plan_for(const seis3d_ctx& c);
-std::vector<float> make_coefficients(const seis3d_ctx& c, int axis);
+// CPML tables [half][n][a, b, 1/kappa] of one axis: the reference profile, or with `profile`
+// (SEIS3D_PROFILE_PARAMS values, checked by check_profile) the learned one.
+std::vector<float> make_coefficients(const seis3d_ctx& c, int axis, const double* profile = nullptr);
+void check_profile(const double* profile);
 float ricker(float frequency, double time);
 
 }  // namespace seis3d::host
@@ -116,11 +119,15 @@ struct seis3d_session {
     float* host_values = nullptr;      // pinned
     int receiver_count = 0;
     float* gather = nullptr;           // [sample][receiver][3]
+    float* stage = nullptr;            // gx*gy*gz floats: field transfers, allocated on first use
     int gather_target = 0, gather_count = 0, gather_stride = 1, gather_receivers = 0;
     bool recording = false;
     bool ghosts_valid = true;          // σ ghost planes consistent with the interior
     seis3d::KernelConfig kernels;
     seis3d::host::AbcFrame* abc = nullptr;   // learned absorbing boundary, when attached
+    bool has_profile = false;                // CPML tables follow `profile`, not the reference's
+    double profile[SEIS3D_PROFILE_PARAMS]{};
+    const double* cpml_profile() const { return has_profile ? profile : nullptr; }
     // Batches whose captured graph holds this session's device pointers.  While any
     // exists, calls that free or replace those buffers are refused, and a destroy is
     // deferred until the last such batch is destroyed.
@@ -165,7 +172,8 @@ struct seis3d_session {
         seis3d::host::free_grid(mu); seis3d::host::free_grid(inv_mu);
         for (int a = 0; a < 3; ++a) { (void)hipFree(psi[a]); (void)hipFree(coef[a]); psi[a] = coef[a] = nullptr; }
         (void)hipFree(flags); (void)hipFree(receivers); (void)hipFree(receiver_values); (void)hipFree(gather);
-        flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr;
+        (void)hipFree(stage);
+        flags = nullptr; receivers = nullptr; receiver_values = nullptr; gather = nullptr; stage = nullptr;
         if (host_values) (void)hipHostFree(host_values);
         host_values = nullptr;
         if (stream) (void)hipStreamDestroy(stream);

@@ -208,10 +208,11 @@ def generate(trainer: Trainer, runs: int, size=(48, 48, 48), steps: int = 240, e
 # ---- evaluation ---------------------------------------------------------------------------------------
 
 def reflection(params, model, src, steps: int, every: int, free_surface: bool = True, frequency: float = 25.0,
-               cases=("learned", "none", "cpml8", "cpml20"), log=None) -> dict:
+               cases=("learned", "none", "cpml8", "cpml20"), log=None, profile=None) -> dict:
     """Relative squared velocity error inside the physical volume against a 40-cell CPML run,
-    summed over every `every`-th step: the reflected energy of each boundary.  `log(message)`,
-    if given, is called after each sampled step."""
+    summed over every `every`-th step: the reflected energy of each boundary ("profile": 8-cell
+    CPML with the learned profile, `profile` or the shipped one).  `log(message)`, if given, is
+    called after each sampled step."""
     nz, ny, nx = model[0].shape
     vmax = float(model[0].max())
     ref_c = W.imported_context(nx, ny, nz, max_vp=vmax, pml=WIDE_PML, free_surface=free_surface, frequency=frequency)
@@ -225,6 +226,8 @@ def reflection(params, model, src, steps: int, every: int, free_surface: bool = 
             g.abc_attach(params)
         elif case == "none":
             g.abc_attach(None)
+        elif case == "profile":
+            g.set_cpml_profile(profile if profile is not None else load_profile())
         runs[case] = (c, g)
     err = {case: 0.0 for case in cases}
     norm = 0.0
@@ -249,6 +252,107 @@ def reflection(params, model, src, steps: int, every: int, free_surface: bool = 
     return {case: err[case] / norm for case in cases}
 
 
+# ---- learned CPML profile ---------------------------------------------------------------------------
+#
+# The second learned boundary: 8-cell CPML whose profile shapes (seis3d_set_cpml_profile: sigma scale,
+# sigma/kappa power, kappa max, alpha scale, alpha power) are fitted to wide-CPML runs.  The step
+# kernels are the plain CPML ones, so the per-step cost is exactly that of 8-cell CPML with the
+# reference profile.  Fitted by Nelder-Mead on the mean log10 reflected energy of seeded training
+# cases (GPU rollouts against stored 40-cell CPML references); judged on held-out seeds.
+# Shipped profile:
+#   python -m cmakepro_cuda_seissim_v2_amd.learned_abc profile --out cmakepro_cuda_seissim_v2_amd/data/cpml_profile.json
+PROFILE_PATH = Path(__file__).resolve().parent / "data" / "cpml_profile.json"
+PROFILE_TRAIN_SEEDS = tuple(range(101, 109))
+PROFILE_HELD_OUT_SEEDS = tuple(range(901, 905))
+
+
+def load_profile(path: Path = PROFILE_PATH) -> tuple:
+    import json
+    d = json.loads(Path(path).read_text())
+    p = tuple(float(v) for v in d["profile"])
+    if len(p) != S.PROFILE_PARAMS:
+        raise ValueError(f"{path}: a CPML profile has {S.PROFILE_PARAMS} parameters")
+    return p
+
+
+def profile_case(seed: int, size=(64, 64, 64), steps: int = 360, every: int = 8, free_surface: bool = True) -> dict:
+    """A seeded case: layered random model, a source of random kind, position, direction and
+    frequency (12..30 Hz), and the velocities of a 40-cell CPML run of the same volume inside the
+    physical volume at every `every`-th step (host, float32)."""
+    nx, ny, nz = size
+    rng = np.random.default_rng(seed)
+    model = layered_random_model(nx, ny, nz, seed)
+    f0 = float(rng.uniform(12.0, 30.0))
+    kind = S.EXPLOSION if rng.random() < 0.5 else S.FORCE
+    d = rng.standard_normal(3)
+    d = d / np.linalg.norm(d)
+    src = S.default_source(x=int(rng.integers(nx // 4, 3 * nx // 4)), y=int(rng.integers(ny // 4, 3 * ny // 4)),
+                           z=int(rng.integers(2, nz // 2)), kind=kind, amplitude=1e8, direction=tuple(d))
+    vmax = float(model[0].max())
+    ref_c = W.imported_context(nx, ny, nz, max_vp=vmax, pml=WIDE_PML, free_surface=free_surface, frequency=f0)
+    thin_c = W.imported_context(nx, ny, nz, max_vp=vmax, pml=8, free_surface=free_surface, frequency=f0)
+    ref = session(ref_c, model)
+    snaps = []
+    norm = 0.0
+    for n in range(steps):
+        ref.step(src, n)
+        if (n + 1) % every == 0:
+            frame = [ref.download_field(f)[ref_c.oz:ref_c.oz + nz, ref_c.oy:ref_c.oy + ny, ref_c.ox:ref_c.ox + nx]
+                     .astype(np.float32) for f in (0, 1, 2)]
+            norm += sum(float(np.sum(v.astype(np.float64) ** 2)) for v in frame)
+            snaps.append(frame)
+    ref.release()
+    return {"seed": seed, "model": model, "src": src, "ctx": thin_c, "steps": steps, "every": every,
+            "snaps": snaps, "norm": norm, "frequency": f0}
+
+
+def profile_reflection(profile, case: dict) -> float:
+    """Reflected energy (relative squared velocity error in the physical volume, summed over the
+    case's sampled steps) of 8-cell CPML with `profile` (None: the reference profile)."""
+    c = case["ctx"]
+    nz, ny, nx = case["model"][0].shape
+    g = session(c, case["model"])
+    if profile is not None:
+        g.set_cpml_profile(profile)
+    err = 0.0
+    k = 0
+    for n in range(case["steps"]):
+        g.step(case["src"], n)
+        if (n + 1) % case["every"] == 0:
+            for f in (0, 1, 2):
+                v = g.download_field(f)[c.oz:c.oz + nz, c.oy:c.oy + ny, c.ox:c.ox + nx]
+                err += float(np.sum((v.astype(np.float64) - case["snaps"][k][f]) ** 2))
+            k += 1
+    g.release()
+    return err / case["norm"]
+
+
+def train_profile(cases, start=S.REFERENCE_PROFILE, max_evals: int = 160, log=None):
+    """Nelder-Mead over the 5 profile parameters, minimising the mean log10 reflected energy over
+    `cases`; parameters outside the accepted ranges score +inf.  Returns (profile, history)."""
+    from scipy.optimize import minimize
+
+    lo = np.array([-5.0, 1.0, 1.0, 0.0, 0.1])
+    hi = np.array([5.0, 8.0, 20.0, 10.0, 8.0])
+    history = []
+
+    def objective(x):
+        if np.any(x < lo) or np.any(x > hi):
+            return np.inf
+        e = [profile_reflection(tuple(x), c) for c in cases]
+        score = float(np.mean(np.log10(e)))
+        history.append((tuple(float(v) for v in x), score))
+        if log:
+            log(f"profile eval {len(history)}: {np.round(x, 4).tolist()} mean log10 {score:.4f}")
+        return score
+
+    x0 = np.asarray(start, dtype=np.float64)
+    simplex = [x0] + [x0 + np.eye(5)[i] * step for i, step in enumerate((0.3, 0.6, 0.8, 0.4, 0.6))]
+    res = minimize(objective, x0, method="Nelder-Mead",
+                   options={"maxfev": max_evals, "initial_simplex": np.array(simplex), "xatol": 1e-3, "fatol": 1e-3})
+    return tuple(float(v) for v in res.x), history
+
+
 def main(argv=None) -> int:
     """``python -m cmakepro_cuda_seissim_v2_amd.learned_abc train|reflect ...``"""
     import argparse
@@ -269,6 +373,12 @@ def main(argv=None) -> int:
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=str(WEIGHTS_PATH))
+    q = sub.add_parser("profile", help="fit the learned CPML profile on GPU rollouts against 40-cell CPML runs")
+    q.add_argument("--size", type=int, default=64)
+    q.add_argument("--steps", type=int, default=360)
+    q.add_argument("--every", type=int, default=8)
+    q.add_argument("--evals", type=int, default=160)
+    q.add_argument("--out", default=str(PROFILE_PATH))
     j = sub.add_parser("reflect", help="reflected energy of the learned frame against a 40-cell CPML run")
     j.add_argument("--size", default="300x300x400")
     j.add_argument("--steps", type=int, default=1500)
@@ -298,6 +408,31 @@ def main(argv=None) -> int:
         print(json.dumps({"samples": n, "runs": a.runs, "size": a.size, "steps": steps, "generate_s": round(t_gen, 2),
                           "train_s": round(t_train, 2), "loss_before": before, "loss_after": after,
                           "epoch_loss": [float(x) for x in losses], "out": a.out}), flush=True)
+        return 0
+    if a.cmd == "profile":
+        size = (a.size,) * 3
+        t0 = time.perf_counter()
+        train, held = [], []
+        for group, seeds in ((train, PROFILE_TRAIN_SEEDS), (held, PROFILE_HELD_OUT_SEEDS)):
+            for sd in seeds:
+                group.append(profile_case(sd, size, a.steps, a.every))
+                log(f"profile reference case {sd}: {time.perf_counter() - t0:.1f} s")
+        t_ref = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        best, history = train_profile(train, max_evals=a.evals, log=log)
+        t_fit = time.perf_counter() - t0
+        rows = {}
+        for name, group in (("train", train), ("held_out", held)):
+            rows[name] = [{"seed": c["seed"], "frequency_hz": round(c["frequency"], 2),
+                           "reference_profile": profile_reflection(None, c), "learned_profile": profile_reflection(best, c)}
+                          for c in group]
+        out = {"profile": list(best), "reference_profile": list(S.REFERENCE_PROFILE), "size": a.size, "steps": a.steps,
+               "every": a.every, "evaluations": len(history), "references_s": round(t_ref, 1), "fit_s": round(t_fit, 1),
+               "cases": rows,
+               "held_out_gain": [r["reference_profile"] / r["learned_profile"] for r in rows["held_out"]]}
+        Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+        Path(a.out).write_text(json.dumps(out, indent=1) + "\n")
+        print(json.dumps(out), flush=True)
         return 0
     nx, ny, nz = (int(v) for v in a.size.split("x"))
     params = load_weights(Path(a.weights))

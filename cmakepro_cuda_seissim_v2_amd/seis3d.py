@@ -170,6 +170,10 @@ def library() -> C.CDLL:
             "seis3d_reset": (C.c_int, [vp]),
             "seis3d_refresh_cpml": (C.c_int, [vp, P(Ctx)]),
             "seis3d_read_cpml": (C.c_int, [vp, C.c_int, P(C.c_float), C.c_size_t]),
+            "seis3d_set_cpml_profile": (C.c_int, [vp, P(C.c_double)]),
+            "seis3d_get_cpml_profile": (C.c_int, [vp, P(C.c_double), P(C.c_int)]),
+            "seis3d_cpml_profile_coefficients": (C.c_int, [P(Ctx), P(C.c_double), C.c_int, P(C.c_float),
+                                                           C.c_size_t]),
             "seis3d_upload_plane": (C.c_int, [vp, C.c_int, P(C.c_float)]),
             "seis3d_finish_upload": (C.c_int, [vp]),
             "seis3d_download_plane": (C.c_int, [vp, C.c_int, P(C.c_float)]),
@@ -330,6 +334,28 @@ def makeCoefficients3D(ctx: Ctx, axis: int) -> np.ndarray:
     return out
 
 
+PROFILE_PARAMS = 5
+REFERENCE_PROFILE = (0.0, 3.0, 3.0, 1.0, 1.0)   # makeCoefficients3D's own shapes
+
+
+def _profile_ptr(profile):
+    if profile is None:
+        return None, None
+    p = np.ascontiguousarray(profile, dtype=np.float64)
+    if p.shape != (PROFILE_PARAMS,):
+        raise InvalidArgument("Seis3D: a CPML profile has 5 parameters.")
+    return p, p.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def cpml_profile_coefficients(ctx: Ctx, profile, axis: int) -> np.ndarray:
+    """[half][n][a, b, 1/kappa] of one axis for a learned CPML profile (None: the reference's)."""
+    n = (ctx.gx, ctx.gy, ctx.gz)[axis]
+    out = np.zeros(n * 6, np.float32)
+    keep, ptr = _profile_ptr(profile)
+    _check(library().seis3d_cpml_profile_coefficients(C.byref(ctx), ptr, axis, _fptr(out), out.size))
+    return out
+
+
 def fitSurvey3D(ctx: Ctx, survey: Survey | None = None) -> Survey:
     q = survey if survey is not None else default_survey()
     _check(library().seis3d_fit_survey(C.byref(ctx), C.byref(q)))
@@ -434,6 +460,20 @@ class GPUSimData3D:
     def refresh_cpml(self, ctx: Ctx) -> None:
         _check(library().seis3d_refresh_cpml(self._h, C.byref(ctx)))
         self.ctx = self._ctx_now()
+
+    def set_cpml_profile(self, profile) -> None:
+        """Use a learned CPML profile (5 parameters) for this session's tables; None restores the
+        reference profile."""
+        keep, ptr = _profile_ptr(profile)
+        _check(library().seis3d_set_cpml_profile(self._h, ptr))
+
+    def cpml_profile(self):
+        """(profile parameters, learned?) of this session's CPML tables."""
+        out = np.zeros(PROFILE_PARAMS, np.float64)
+        learned = C.c_int()
+        _check(library().seis3d_get_cpml_profile(self._h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 C.byref(learned)))
+        return out, bool(learned.value)
 
     def read_cpml(self, axis: int) -> np.ndarray:
         n = (self.ctx.gx, self.ctx.gy, self.ctx.gz)[axis]

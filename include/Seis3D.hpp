@@ -266,6 +266,12 @@ inline void refreshCpmlCoefficients3D(GPUSimData3D& g, const SimulationContext3D
     detail::check(seis3d_refresh_cpml(detail::session(g), &o));
 }
 
+// Learned CPML profile (no reference counterpart): SEIS3D_PROFILE_PARAMS shape parameters for
+// this session's CPML tables, nullptr for the reference profile; kept by refreshCpmlCoefficients3D.
+inline void setCpmlProfile3D(GPUSimData3D& g, const double* profile) {
+    detail::check(seis3d_set_cpml_profile(detail::session(g), profile));
+}
+
 // runStep3D (cuda/Cuda_Seis3D.cu:602-624).
 inline void runStep3D(GPUSimData3D& g, const SimulationContext3D&, const Source3D& s, std::uint64_t step) {
     const seis3d_source o = detail::toC(s);

@@ -158,6 +158,26 @@ int seis3d_refresh_cpml(seis3d_t* t, const seis3d_ctx* ctx);
 /* Read back the device CPML table of one axis ([half][n][a,b,1/kappa]). */
 int seis3d_read_cpml(seis3d_t* t, int axis, float* out, size_t capacity);
 
+/* Learned CPML profile.  No counterpart in the reference, whose profile is fixed
+ * (makeCoefficients3D, cuda/Cuda_Seis3D.cu:45-64: sigma = sigma_max d^3 with
+ * sigma_max = -4 vp_max ln(1e-6) / (2 pml h), kappa = 1 + 2 d^3, alpha = pi f (1 - d)
+ * at layer depth d in [0, 1]).  The learned profile keeps the table layout and the
+ * a/b/kappa formulas and replaces the three shapes by
+ *     sigma = e^p[0] sigma_max d^p[1],  kappa = 1 + (p[2] - 1) d^p[1],
+ *     alpha = p[3] pi f (1 - d)^p[4]   (alpha = 0 at d = 0, as the reference),
+ * so the step costs exactly what the reference profile costs.  The reference
+ * profile is p = {0, 3, 3, 1, 1}.  Accepted ranges: |p[0]| <= 5, 1 <= p[1] <= 8,
+ * 1 <= p[2] <= 20, 0 <= p[3] <= 10, 0.1 <= p[4] <= 8; anything else is
+ * SEIS3D_EINVAL.  seis3d_set_cpml_profile uploads the tables for the session (NULL
+ * restores the reference profile); seis3d_refresh_cpml keeps the session's profile.
+ * Refused (SEIS3D_ELIFECYCLE) while a batch holds the session or a learned frame
+ * is attached; a no-op in reflecting mode. */
+#define SEIS3D_PROFILE_PARAMS 5
+int seis3d_set_cpml_profile(seis3d_t* t, const double* profile);
+int seis3d_get_cpml_profile(const seis3d_t* t, double* profile, int* learned);
+int seis3d_cpml_profile_coefficients(const seis3d_ctx* ctx, const double* profile, int axis, float* out,
+                                     size_t capacity);
+
 /* ---- Model ---------------------------------------------------------------- */
 
 /* uploadModelPlane3D (cuda/Cuda_Seis3D.cu:696-711). */

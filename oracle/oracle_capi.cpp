@@ -93,6 +93,17 @@ void oracle3d_destroy(Handle* h) { delete h; }
 
 int oracle3d_reset(Handle* h) { return guard([&] { with_solver(h, [](auto& s) { s.reset(); }); }); }
 
+int oracle3d_set_profile(Handle* h, const double* profile) {
+    return guard([&] { with_solver(h, [&](auto& s) { s.set_profile(profile); }); });
+}
+
+int oracle3d_profile_coefficients(const seis3d_ctx* ctx, const double* profile, int axis, float* out) {
+    return guard([&] {
+        const auto v = seis3d_oracle::make_profile_coefficients<float>(*ctx, axis, profile);
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
 int oracle3d_refresh_cpml(Handle* h, const seis3d_ctx* ctx) {
     return guard([&] { with_solver(h, [&](auto& s) { s.refresh_cpml(*ctx); }); });
 }

@@ -47,6 +47,8 @@ def library() -> C.CDLL:
             "oracle3d_destroy": (None, [vp]),
             "oracle3d_reset": (C.c_int, [vp]),
             "oracle3d_refresh_cpml": (C.c_int, [vp, P(Ctx)]),
+            "oracle3d_set_profile": (C.c_int, [vp, P(C.c_double)]),
+            "oracle3d_profile_coefficients": (C.c_int, [P(Ctx), P(C.c_double), C.c_int, P(C.c_float)]),
             "oracle3d_upload_plane": (C.c_int, [vp, C.c_int, P(C.c_float)]),
             "oracle3d_finish_upload": (C.c_int, [vp]),
             "oracle3d_set_receivers": (C.c_int, [vp, P(Receiver), C.c_int]),
@@ -109,6 +111,16 @@ def coefficients(ctx: Ctx, axis: int) -> np.ndarray:
     return out
 
 
+def profile_coefficients(ctx: Ctx, profile, axis: int) -> np.ndarray:
+    """Learned-profile CPML table of one axis, [half][n][a, b, 1/kappa] (FP32)."""
+    n = (ctx.gx, ctx.gy, ctx.gz)[axis]
+    out = np.zeros(n * 6, np.float32)
+    p = np.ascontiguousarray(profile, dtype=np.float64)
+    _check(library().oracle3d_profile_coefficients(C.byref(ctx), _dptr(p), axis,
+                                                   out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
 def slab_count(ctx: Ctx, axis: int) -> int:
     return int(library().oracle3d_slab_count(C.byref(ctx), axis))
 
@@ -160,6 +172,14 @@ class Oracle3D:
 
     def refresh_cpml(self, ctx: Ctx) -> None:
         _check(library().oracle3d_refresh_cpml(self._h, C.byref(ctx)))
+
+    def set_profile(self, profile) -> None:
+        """Learned CPML profile tables (None: the reference profile)."""
+        if profile is None:
+            _check(library().oracle3d_set_profile(self._h, None))
+            return
+        p = np.ascontiguousarray(profile, dtype=np.float64)
+        _check(library().oracle3d_set_profile(self._h, _dptr(p)))
 
     def upload_plane(self, z: int, vp_vs_rho: np.ndarray) -> None:
         a = np.ascontiguousarray(vp_vs_rho, dtype=np.float32)

@@ -131,6 +131,37 @@ std::vector<Real> make_coefficients(const seis3d_ctx& c, int axis) {
     return values;
 }
 
+// A learned CPML profile p = {log sigma scale, power, kappa max, alpha scale, alpha power} in the
+// reference's table (no counterpart in the reference; its own profile is p = {0, 3, 3, 1, 1}):
+// the layer depth of makeCoefficients3D (cuda/Cuda_Seis3D.cu:49-53) and its a/b formulas
+// (:57-58), with sigma = e^p0 sigma_max depth^p1, kappa = 1 + (p2 - 1) depth^p1 and
+// alpha = p3 pi f (1 - depth)^p4 for depth > 0.
+template <class Real>
+std::vector<Real> make_profile_coefficients(const seis3d_ctx& c, int axis, const double* p) {
+    const int n = axis == 0 ? c.gx : axis == 1 ? c.gy : c.gz;
+    std::vector<Real> table(std::size_t(n) * 6);
+    const double pi = 3.141592653589793;
+    const double reference_sigma = -4.0 * c.max_vp * std::log(1.0e-6) / (2.0 * c.pml * c.spacing);
+    const double top = std::exp(p[0]) * reference_sigma;
+    for (int half = 0; half < 2; ++half)
+        for (int i = 0; i < n; ++i) {
+            // Distance into the layer in layer widths, from whichever side is closer.
+            const double x = i + 0.5 * half;
+            double d = (x - (n - kRadius - c.pml - 0.5)) / c.pml;
+            if (!(axis == 2 && c.free_surface)) d = std::max(d, (kRadius + c.pml - 0.5 - x) / c.pml);
+            d = std::clamp(d, 0.0, 1.0);
+            const double shape = std::pow(d, p[1]);
+            const double sigma = top * shape;
+            const double kappa = 1.0 + (p[2] - 1.0) * shape;
+            const double alpha = d > 0 ? p[3] * pi * c.frequency * std::pow(1.0 - d, p[4]) : 0.0;
+            const double b = std::exp(-(sigma / kappa + alpha) * c.dt);
+            const double a = sigma > 0 ? sigma * (b - 1.0) / (kappa * (sigma + kappa * alpha)) : 0.0;
+            Real* row = table.data() + (std::size_t(half) * n + i) * 3;
+            row[0] = Real(a); row[1] = Real(b); row[2] = Real(1.0 / kappa);
+        }
+    return table;
+}
+
 // sourceKernel3D time function (cuda/Cuda_Seis3D.cu:319-320), FP64 phase.
 inline double ricker_double(double frequency, double time) {
     const double phase = 3.141592653589793 * frequency * (time - 1.2 / frequency);
@@ -160,6 +191,8 @@ public:
     std::vector<Real> coef[3];      // [half][n][a,b,1/kappa]
     std::vector<seis3d_receiver> receivers;
     bool nonfinite = false;
+    bool learned = false;           // CPML tables from `profile` (set_profile)
+    double profile[5]{};
 
     // initializeGpu3D (cuda/Cuda_Seis3D.cu:519-555) minus device work: all
     // fields zero, CPML tables uploaded, procedural materials generated.
@@ -185,11 +218,19 @@ public:
         nonfinite = false;
     }
 
-    // refreshCpmlCoefficients3D (cuda/Cuda_Seis3D.cu:580-599).
+    // refreshCpmlCoefficients3D (cuda/Cuda_Seis3D.cu:580-599); a learned profile is kept.
     void refresh_cpml(const seis3d_ctx& ctx) {
         if (c.reflecting) return;
-        for (int a = 0; a < 3; ++a) coef[a] = make_coefficients<Real>(ctx, a);
+        for (int a = 0; a < 3; ++a)
+            coef[a] = learned ? make_profile_coefficients<Real>(ctx, a, profile) : make_coefficients<Real>(ctx, a);
         c = ctx;
+    }
+
+    // Tables of a learned CPML profile (nullptr: the reference's).
+    void set_profile(const double* p) {
+        learned = p != nullptr;
+        if (p) std::copy(p, p + 5, profile);
+        if (!c.reflecting) refresh_cpml(c);
     }
 
     // generateMaterialsKernel3D (cuda/Cuda_Seis3D.cu:114-147).  The blend is
