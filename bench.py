@@ -84,16 +84,17 @@ def measured_ceilings() -> dict:
     return out
 
 
-def profiled_traffic(kernel: str, grid: str):
+def profiled_traffic(kernel: str, grid: str, tiles):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload
-    (profiles/<family>_kernels_<grid>.md, line 'traffic_bytes_per_launch: name=bytes ...')."""
+    (profiles/<family>_kernels_<grid>.md, line 'traffic_bytes_per_launch: tiles=<tile_y>x<z_chunk>
+    name=bytes ...'), when it was measured with the same kernel shape."""
+    shape = f"{tiles[1]}x{tiles[2]}"
     for path in sorted((ROOT / "profiles").glob(f"*_{grid}.md")):
         for line in path.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
-                for item in line.split(":", 1)[1].split():
-                    name, _, value = item.partition("=")
-                    if name == kernel:
-                        return int(value), path.name
+                items = dict(item.partition("=")[::2] for item in line.split(":", 1)[1].split())
+                if items.get("tiles") == shape and kernel in items:
+                    return int(items[kernel]), path.name
     return None, None
 
 
@@ -546,14 +547,15 @@ def cpu_baseline(ctx, model, src, rec, steps_hint, n_psi):
     return baseline, snapshot, done - 3
 
 
-def snapshot_parity(ctx, model, src, oracle32, steps):
+def snapshot_parity(ctx, model, src, oracle32, steps, tiles):
     """The bench line's own error: the 9 wavefields after `steps` steps of a fresh GPU session
-    (the bench's kernels) against the FP32 oracle state of the CPU baseline and an FP64 oracle
-    run, rel-L2 over the updated cells, worst field."""
+    (the bench's kernels, shape `tiles`) against the FP32 oracle state of the CPU baseline and an
+    FP64 oracle run, rel-L2 over the updated cells, worst field."""
     import numpy as np
     from cmakepro_cuda_seissim_v2_amd import seis3d as S
     from oracle import pyoracle as O
     g = S.GPUSimData3D(ctx)
+    g.set_tiles(*tiles)
     g.upload_model(*model)
     g.run(src, 0, steps)
     gpu = [g.download_field(f) for f in range(9)]
@@ -648,7 +650,7 @@ def main():
             "grid": [nx, ny, nz], "padded": [ctx.gx, ctx.gy, ctx.gz],
             "n_upd": n_upd, "n_psi": sb["n_psi"], "dt_us": round(ctx.dt * 1e6),
             "kernel_family": FAMILY_NAMES.get(family, str(family)),
-            "tile_y": tile_y, "z_chunk": z_chunk,
+            "tile_y": tile_y, "z_chunk": z_chunk, "autotuned": run["autotuned"],
             "step_hbm_gbs_algorithmic": round(step_gbs, 1),
             "step_frac_of_peak": round(step_gbs / HBM_PEAK_GBS, 4),
             "velocity_kernel_us": round(vel_us, 2), "stress_kernel_us": round(str_us, 2),
@@ -674,7 +676,7 @@ def main():
         out["roofline"]["measured_ceilings_gbs"] = ceilings
         if ceilings.get("copy"):
             out["roofline"]["frac_of_copy_ceiling"] = round(dominant[3] / ceilings["copy"], 4)
-    tb, src_file = profiled_traffic(out["roofline"]["kernel"], args.grid)
+    tb, src_file = profiled_traffic(out["roofline"]["kernel"], args.grid, run["tiles"])
     if tb:
         out["roofline"]["traffic"] = tb
         out["roofline"]["traffic_gbs"] = round(tb / (dominant[1] * 1e-6) / 1e9, 1)
@@ -682,7 +684,7 @@ def main():
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"], snapshot, snap_steps = cpu_baseline(ctx, model, src, rec, args.cpu_steps, sb["n_psi"])
-        out["parity"] = snapshot_parity(ctx, model, src, snapshot, snap_steps)
+        out["parity"] = snapshot_parity(ctx, model, src, snapshot, snap_steps, run["tiles"])
     if not args.no_extra:
         out["grid_200"] = grid_200(args, S, W)
         out["fwi"] = fwi_leg(S, W)
@@ -692,13 +694,18 @@ def main():
 
 
 def step_run(args, S, ctx, model, src, rec, steps, warmup):
-    """Time `steps` steps (after `warmup`) with the default kernels, gathers recorded on the device;
-    then per-kernel durations with HIP events on the session stream."""
+    """Time `steps` steps (after `warmup`), gathers recorded on the device, with the kernel shape the
+    library's autotuner picks for this grid (or the one given on the command line); then per-kernel
+    durations with HIP events on the session stream."""
     sb = S.step_bytes(ctx)
     g = S.GPUSimData3D(ctx)
+    tuned = None
     if args.tile_y or args.z_chunk or args.family >= 0:
         f0, ty0, zc0 = g.get_tiles()
         g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk or zc0)
+    else:
+        tuned = g.autotune(src, steps=20)   # times every candidate on a scratch session, keeps the best
+        print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
     tiles = g.get_tiles()
     g.upload_model(*model)
     g.set_receivers(rec)
@@ -718,8 +725,8 @@ def step_run(args, S, ctx, model, src, rec, steps, warmup):
     kt = g.run_timed(src, total_steps, args.kernel_steps)
     g.synchronize()
     g.release()
-    return {"bytes": sb, "secs": secs, "tiles": tiles, "velocity_us": kt.velocity_ms * 1e3 / kt.launches,
-            "stress_us": kt.stress_ms * 1e3 / kt.launches}
+    return {"bytes": sb, "secs": secs, "tiles": tiles, "autotuned": tuned is not None,
+            "velocity_us": kt.velocity_ms * 1e3 / kt.launches, "stress_us": kt.stress_ms * 1e3 / kt.launches}
 
 
 def grid_200(args, S, W) -> dict:
@@ -736,6 +743,7 @@ def grid_200(args, S, W) -> dict:
     sb = run["bytes"]
     snap = 41
     g = S.GPUSimData3D(ctx)
+    g.set_tiles(*run["tiles"])
     g.upload_model(*model)
     g.run(src, 0, snap)
     core = (slice(4, ctx.gz - 4), slice(4, ctx.gy - 4), slice(4, ctx.gx - 4))
@@ -754,6 +762,8 @@ def grid_200(args, S, W) -> dict:
         "n_upd": sb["n_upd"], "n_psi": sb["n_psi"], "steps": steps,
         "gcell_updates_per_s": round(sb["n_upd"] * steps / run["secs"] / 1e9, 4),
         "ms_per_step": round(run["secs"] / steps * 1e3, 5),
+        "kernel_family": FAMILY_NAMES.get(run["tiles"][0], str(run["tiles"][0])), "tile_y": run["tiles"][1],
+        "z_chunk": run["tiles"][2], "autotuned": run["autotuned"],
         "velocity_kernel_us": round(run["velocity_us"], 2), "stress_kernel_us": round(run["stress_us"], 2),
         "velocity_frac_of_peak": round(sb["velocity"] / (run["velocity_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         "stress_frac_of_peak": round(sb["stress"] / (run["stress_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),

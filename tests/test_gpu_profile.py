@@ -136,3 +136,44 @@ def test_shipped_profile_beats_reference_cpml_on_held_out_cases(gpu):
             f"gain {ref / got:.2f}")
         gains.append(ref / got)
     assert min(gains) > 1.5
+
+
+def test_fwi_gradient_with_learned_profile_matches_oracle(gpu):
+    """The adjoint reads the session's CPML tables, so a learned profile carries through the FWI
+    gradient: it matches the FP32 and FP64 oracle adjoints with the same profile, and differs from
+    the gradient with the reference profile by far more than from the FP32 oracle."""
+    import fwi_cases as F
+    nx, ny, nz, steps = 18, 16, 20, 160
+    learned = PROFILES[1]
+    start, true = F.models(nx, ny, nz)
+    ctx = F.context(nx, ny, nz, true, free_surface=True)
+    src = F.source(S.EXPLOSION, nx, ny, nz)
+    rec = F.receivers(nx, ny, nz, surface=True)
+    with O.Oracle3D(ctx, 64) as o:
+        o.upload_model(*true)
+        o.set_profile(learned)
+        o.set_receivers(rec)
+        obs = o.run(src, 0, steps, stride=1, max_samples=steps).astype(np.float32)
+    grads = {}
+    for name, profile in (("learned", learned), ("reference", None)):
+        with S.GPUSimData3D(ctx) as g:
+            g.upload_model(*start)
+            g.set_cpml_profile(profile)
+            grads[name] = g.fwi_gradient([(src, rec, obs)], steps, 1, 10)
+    misfit, *grad = grads["learned"]
+    err32 = 0.0
+    for precision in (32, 64):
+        with O.Oracle3D(ctx, precision) as o:
+            o.upload_model(*start)
+            o.set_profile(learned)
+            o.set_receivers(rec)
+            m_ref, *g_ref = o.fwi_gradient(src, steps, 1, obs.astype(np.float64))
+        errs = [rel_l2(a, b) for a, b in zip(grad, g_ref)]
+        log(f"learned-profile FWI vs FP{precision} oracle: misfit {misfit:.6e} / {m_ref:.6e}, rel-L2 {errs}")
+        assert misfit == pytest.approx(m_ref, rel=1e-4)
+        assert max(errs) < 1e-4
+        if precision == 32:
+            err32 = max(errs)
+    shift = min(rel_l2(a, b) for a, b in zip(grads["reference"][1:], grad))
+    log(f"reference-profile gradient differs by {shift:.3e} (FP32 oracle: {err32:.3e})")
+    assert shift > 30 * err32
