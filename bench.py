@@ -37,7 +37,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--family", type=int, default=-1, help="kernel family (-1: library default)")
     p.add_argument("--tile-y", type=int, default=0)
-    p.add_argument("--z-chunk", type=int, default=0)
+    p.add_argument("--z-chunk", type=int, default=-1,
+                   help="planes per workgroup (0: balanced one-round chunks with --tile-y 8; default: autotuned)")
     p.add_argument("--kernel-steps", type=int, default=200, help="steps timed per kernel with HIP events")
     p.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     p.add_argument("--no-cpu", action="store_true")
@@ -700,9 +701,9 @@ def step_run(args, S, ctx, model, src, rec, steps, warmup):
     sb = S.step_bytes(ctx)
     g = S.GPUSimData3D(ctx)
     tuned = None
-    if args.tile_y or args.z_chunk or args.family >= 0:
+    if args.tile_y or args.z_chunk >= 0 or args.family >= 0:
         f0, ty0, zc0 = g.get_tiles()
-        g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk or zc0)
+        g.set_tiles(args.family if args.family >= 0 else f0, args.tile_y or ty0, args.z_chunk if args.z_chunk >= 0 else zc0)
     else:
         tuned = g.autotune(src, steps=20)   # times every candidate on a scratch session, keeps the best
         print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)

@@ -1039,8 +1039,8 @@ int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk) {
         require_session(t);
         if (family != 0 && family != 4) invalid("Seis3D: kernel family must be 0 (direct) or 4 (LDS-DMA).");
         if (family == 4 && tile_y != 4 && tile_y != 8) invalid("Seis3D: tile_y must be 4 or 8.");
-        if (family == 4 && (z_chunk < 1 || z_chunk > 128))
-            invalid("Seis3D: z_chunk must be 1..128 for the LDS-DMA family.");
+        if (family == 4 && (z_chunk < 0 || z_chunk > 128 || (z_chunk == 0 && tile_y != 8)))
+            invalid("Seis3D: z_chunk must be 1..128 for the LDS-DMA family (0: balanced one-round chunks, tile_y 8).");
         t->kernels.family = family;
         if (family >= 1) {
             t->kernels.tile_y = tile_y;
@@ -1176,6 +1176,7 @@ std::vector<seis3d::KernelConfig> tile_candidates(const seis3d_ctx& c) {
         for (int zc : chunks)
             if (zc >= 8 && zc <= uz && zc <= 128) out.push_back({4, ty, zc});
     }
+    out.push_back({4, 8, 0});   // one round, chunk counts balanced across the tile columns
     return out;
 }
 

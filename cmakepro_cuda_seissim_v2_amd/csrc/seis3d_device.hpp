@@ -181,6 +181,15 @@ __device__ __forceinline__ void shift(float (&q)[N], float lead) {
     q[N - 1] = lead;
 }
 
+// A one-round launch whose tile columns have their own chunk counts (balanced_map in
+// seis3d_dma.hip): item k = bx | by << 6 | chunk << 14 | chunks << 22, dealt like tile_of.
+constexpr int kBalancedItems = 512;
+struct BalancedMap {
+    int nblocks;
+    int per_xcd;           // launched blocks / 8
+    unsigned item[kBalancedItems];
+};
+
 // Several shots' tiles in one launch, passed by value as the kernel argument, so that every
 // shot's parameters are read from the kernel-argument segment exactly as a single shot's are
 // (constant memory: its field pointers are known to address global memory).  Shot s owns

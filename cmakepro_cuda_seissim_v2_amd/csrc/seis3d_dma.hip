@@ -19,6 +19,8 @@
 // The loads are explicit waits rather than compiler-tracked, so every wait
 // is conservative: a wave may wait for more than it needs, never less.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "seis3d_device.hpp"
 #include "seis3d_kernels.hpp"
@@ -716,6 +718,34 @@ __global__ __launch_bounds__(64 * BY) void stress_dma(DevStep P, TileMap M) {
     stress_tile<BY, OOP>(sh, P, M, bx, by, bz);
 }
 
+// One-round launches whose tile columns get different chunk counts (balanced_map below): item k
+// of the map packs the column (bx, by), the chunk index and the column's chunk count, and the tile
+// body takes the chunk bounds from a map with that count.
+template <int BY>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void velocity_dma_balanced(DevStep P,
+                                                                                                    BalancedMap B) {
+    __shared__ VelShared<BY> sh;
+    const int b = int(blockIdx.x);
+    const int logical = (b & 7) * B.per_xcd + (b >> 3);
+    if (logical >= B.nblocks) return;
+    const unsigned it = B.item[logical];
+    TileMap M{};
+    M.nbz = int(it >> 22) & 255;
+    velocity_tile<BY, false>(sh, P, M, int(it & 63u), int(it >> 6) & 255, int(it >> 14) & 255);
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void stress_dma_balanced(DevStep P, BalancedMap B) {
+    __shared__ StrShared<BY> sh;
+    const int b = int(blockIdx.x);
+    const int logical = (b & 7) * B.per_xcd + (b >> 3);
+    if (logical >= B.nblocks) return;
+    const unsigned it = B.item[logical];
+    TileMap M{};
+    M.nbz = int(it >> 22) & 255;
+    stress_tile<BY, false>(sh, P, M, int(it & 63u), int(it >> 6) & 255, int(it >> 14) & 255);
+}
+
 // The same half-steps for several shots in one launch: the shots' tiles are dealt out as one
 // list (shot s owns logical tiles [L.first[s], L.first[s + 1])), XCD-aware like a single shot.
 // L.steps[s] is shot s's step parameters for this launch (its own grid, fields and source term).
@@ -882,7 +912,68 @@ hipError_t dma_residency(int tile_y, int* blocks_per_cu, int* cus) {
     return e;
 }
 
+// A balanced one-round map.  A tile column whose cells lie in an x or a y CPML slab costs more per
+// plane than an interior one (its CPML memories are read and written, and a tile that the slab
+// ends mid-way costs a whole tile), so with equal chunks the interior columns finish early and
+// idle while the slab columns finish the launch.  Here each column class gets its own chunk count,
+// in proportion to its cost per plane, so that every workgroup of the one round carries about the
+// same work: the smallest per-workgroup work T for which the columns' chunk counts
+// ceil(planes * weight / T) fit the resident slots.  Items are ordered by the depth of their
+// chunk, then row, then column, and dealt to the XCDs in contiguous ranges (as tile_of deals the
+// uniform map), so each XCD takes a band of depths and neighbouring tiles share its L2.
+// Returns false when no balanced map fits one round (too many columns or chunks over 128 planes).
+bool balanced_map(const DevGrid& g, int tile_y, int slots, BalancedMap* out) {
+    constexpr double kSlabX = 0.21, kSlabY = 0.16;   // extra cost per plane, measured per class
+    const int nbx = (g.gx - 8 + 63) / 64, nby = (g.gy - 8 + tile_y - 1) / tile_y, uz = g.gz - 8;
+    const int cols = nbx * nby;
+    slots = std::min(slots, kBalancedItems);
+    const int min_chunks = (uz + kMaxChunk - 1) / kMaxChunk;
+    if (cols < 1 || nbx > 64 || nby > 256 || cols * min_chunks > slots) return false;
+    std::vector<double> weight(cols);
+    for (int by = 0; by < nby; ++by)
+        for (int bx = 0; bx < nbx; ++bx) {
+            const int x0 = 4 + bx * 64, x1 = std::min(x0 + 64, g.gx - 4);
+            const int y0 = 4 + by * tile_y, y1 = std::min(y0 + tile_y, g.gy - 4);
+            const bool xs = g.pml > 0 && (x0 < 4 + g.pml || x1 > g.gx - 4 - g.pml);
+            const bool ys = g.pml > 0 && (y0 < 4 + g.pml || y1 > g.gy - 4 - g.pml);
+            weight[by * nbx + bx] = 1.0 + (xs ? kSlabX : 0.0) + (ys ? kSlabY : 0.0);
+        }
+    const int max_chunks = std::max(min_chunks, std::min(255, uz / 8));   // chunks of 8 planes or more
+    auto chunks = [&](double work, int c) {
+        return std::min(max_chunks, std::max(min_chunks, int(std::ceil(uz * weight[c] / work))));
+    };
+    auto total = [&](double work) {
+        int n = 0;
+        for (int c = 0; c < cols; ++c) n += chunks(work, c);
+        return n;
+    };
+    double lo = 0.5, hi = 2.0 * uz;                  // total(hi) = cols * min_chunks <= slots
+    for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (total(mid) <= slots) hi = mid; else lo = mid;
+    }
+    struct Item { double depth; int by, bx, j, n; };
+    std::vector<Item> items;
+    for (int c = 0; c < cols; ++c) {
+        const int n = chunks(hi, c);
+        for (int j = 0; j < n; ++j) items.push_back({(j + 0.5) / n, c / nbx, c % nbx, j, n});
+    }
+    if (int(items.size()) > slots) return false;
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        if (a.depth != b.depth) return a.depth < b.depth;
+        return a.by != b.by ? a.by < b.by : a.bx < b.bx;
+    });
+    out->nblocks = int(items.size());
+    out->per_xcd = (out->nblocks + 7) / 8;
+    for (int k = 0; k < out->nblocks; ++k) {
+        const Item& i = items[k];
+        out->item[k] = unsigned(i.bx) | unsigned(i.by) << 6 | unsigned(i.j) << 14 | unsigned(i.n) << 22;
+    }
+    return true;
+}
+
 TileMap dma_tile_map(const DevGrid& g, int tile_y, int zchunk) {
+    if (zchunk <= 0) zchunk = 16;   // balanced launches (zchunk 0) step batches with the default chunks
     TileMap m;
     m.nbx = (g.gx - 8 + 63) / 64;
     m.nby = (g.gy - 8 + tile_y - 1) / tile_y;
@@ -913,9 +1004,32 @@ hipError_t launch_dma_shots(bool velocity, const ShotLaunch& L, int tile_y, hipS
 
 hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, hipStream_t s) {
     if (zchunk > kMaxChunk) return hipErrorInvalidValue;
+    const bool oop = out_of_place(P.p);
+    if (zchunk == 0 && tile_y == 8 && !oop) {
+        // The balanced map of this grid, built once per thread and grid.
+        thread_local BalancedMap cached{};
+        thread_local int key[6] = {-1, -1, -1, -1, -1, -1};
+        thread_local bool ok = false;
+        const int k[6] = {P.g.gx, P.g.gy, P.g.gz, P.g.pml, tile_y, 0};
+        if (!std::equal(k, k + 6, key)) {
+            int dev = 0, cus = 0, v = 0, st = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma_balanced<8>, 64 * 8, 0);
+            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma_balanced<8>, 64 * 8, 0);
+            if (e != hipSuccess) return e;
+            ok = balanced_map(P.g, tile_y, std::min(v, st) * cus, &cached);
+            std::copy(k, k + 6, key);
+        }
+        if (ok) {
+            const dim3 grid(unsigned(cached.per_xcd) * 8);
+            if (velocity) hipLaunchKernelGGL(velocity_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, cached);
+            else hipLaunchKernelGGL(stress_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, cached);
+            return hipGetLastError();
+        }
+    }
     const TileMap m = dma_tile_map(P.g, tile_y, zchunk);
     const dim3 grid(unsigned(m.per_xcd) * 8);
-    const bool oop = out_of_place(P.p);
     switch (tile_y) {
     case 4:
         if (oop) {

@@ -273,7 +273,10 @@ int seis3d_extract_views_noalloc(seis3d_t* t, float* out, size_t capacity, const
 
 /* Kernel family: 0 = direct (one thread per cell), 4 = 2.5D LDS-DMA streaming
  * (the default).  tile_y rows (4 or 8) and z_chunk planes (1..128) per
- * workgroup of family 4. */
+ * workgroup of family 4.  z_chunk 0 with tile_y 8: one round of resident
+ * workgroups, each tile column with its own chunk count so that columns in CPML
+ * slabs (more work per plane) get shorter chunks; where that does not fit one
+ * round, and for out-of-place and batched steps, 16-plane chunks. */
 int seis3d_set_tiles(seis3d_t* t, int family, int tile_y, int z_chunk);
 int seis3d_get_tiles(const seis3d_t* t, int* family, int* tile_y, int* z_chunk);
 /* seis3d_run with HIP events around every fused-kernel launch. */

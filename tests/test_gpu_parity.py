@@ -61,7 +61,7 @@ CASES = {
         dict(x=12, y=10, z=11, kind=S.EXPLOSION), None, 100, 2),
 }
 
-TILES = [(0, 8, 32), (4, 4, 5), (4, 4, 16), (4, 4, 37), (4, 8, 7), (4, 8, 16), (4, 8, 128)]
+TILES = [(0, 8, 32), (4, 4, 5), (4, 4, 16), (4, 4, 37), (4, 8, 7), (4, 8, 16), (4, 8, 128), (4, 8, 0)]
 
 
 def receivers_for(c):
@@ -191,7 +191,7 @@ def test_stream_equals_direct_bitwise(gpu):
     model = W.layered_model(c.nx, c.ny, c.nz, seed=9)
     src = W.config2_source(c)
     out = []
-    for tiles in [(0, 8, 32), (4, 4, 7), (4, 4, 16), (4, 8, 20), (4, 8, 128)]:
+    for tiles in [(0, 8, 32), (4, 4, 7), (4, 4, 16), (4, 8, 20), (4, 8, 128), (4, 8, 0)]:
         g = S.GPUSimData3D(c)
         g.set_tiles(*tiles)
         g.upload_model(*model)
