@@ -1006,25 +1006,36 @@ hipError_t launch_dma(bool velocity, const DevStep& P, int tile_y, int zchunk, h
     if (zchunk > kMaxChunk) return hipErrorInvalidValue;
     const bool oop = out_of_place(P.p);
     if (zchunk == 0 && tile_y == 8 && !oop) {
-        // The balanced map of this grid, built once per thread and grid.
-        thread_local BalancedMap cached{};
-        thread_local int key[6] = {-1, -1, -1, -1, -1, -1};
-        thread_local bool ok = false;
-        const int k[6] = {P.g.gx, P.g.gy, P.g.gz, P.g.pml, tile_y, 0};
-        if (!std::equal(k, k + 6, key)) {
+        // The balanced maps of the grids this thread launches (batches alternate between shots of
+        // several grids), each built once.
+        struct Entry {
+            int key[5];
+            bool ok;
+            BalancedMap map;
+        };
+        thread_local std::vector<Entry> cache;
+        const int k[5] = {P.g.gx, P.g.gy, P.g.gz, P.g.pml, tile_y};
+        const Entry* hit = nullptr;
+        for (const auto& e : cache)
+            if (std::equal(k, k + 5, e.key)) { hit = &e; break; }
+        if (!hit) {
             int dev = 0, cus = 0, v = 0, st = 0;
             hipError_t e = hipGetDevice(&dev);
             if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
             if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, velocity_dma_balanced<8>, 64 * 8, 0);
             if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&st, stress_dma_balanced<8>, 64 * 8, 0);
             if (e != hipSuccess) return e;
-            ok = balanced_map(P.g, tile_y, std::min(v, st) * cus, &cached);
-            std::copy(k, k + 6, key);
+            if (cache.size() >= 64) cache.erase(cache.begin());
+            Entry fresh{};
+            std::copy(k, k + 5, fresh.key);
+            fresh.ok = balanced_map(P.g, tile_y, std::min(v, st) * cus, &fresh.map);
+            cache.push_back(fresh);
+            hit = &cache.back();
         }
-        if (ok) {
-            const dim3 grid(unsigned(cached.per_xcd) * 8);
-            if (velocity) hipLaunchKernelGGL(velocity_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, cached);
-            else hipLaunchKernelGGL(stress_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, cached);
+        if (hit->ok) {
+            const dim3 grid(unsigned(hit->map.per_xcd) * 8);
+            if (velocity) hipLaunchKernelGGL(velocity_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, hit->map);
+            else hipLaunchKernelGGL(stress_dma_balanced<8>, grid, dim3(64, 8), 0, s, P, hit->map);
             return hipGetLastError();
         }
     }
