@@ -60,8 +60,10 @@ def log(msg):
     sys.stdout.flush()
 
 
-def gpu_run(ctx, model, src, rec, steps, stride):
+def gpu_run(ctx, model, src, rec, steps, stride, tiles=None):
     g = S.GPUSimData3D(ctx)
+    if tiles:
+        g.set_tiles(*tiles)
     g.upload_model(*model)
     g.set_receivers(rec)
     g.gather_begin(steps // stride, stride)
@@ -103,7 +105,19 @@ def test_config2_full_length_parity(gpu):
     assert g.get_tiles()[0] == 4
     assert gather.shape == (500, 150, 3) and np.all(np.isfinite(gather))
     oracle_parity(ctx, model, src, rec, steps, stride, gather, g, "config 2")
+    same_as_balanced(ctx, model, src, rec, steps, stride, gather, g, "config 2")
     g.release()
+
+
+def same_as_balanced(ctx, model, src, rec, steps, stride, gather, g, name):
+    """The bench's autotuned shape (64x8 tiles, one round, balanced chunk counts) steps the same
+    bits as the default shape over the whole run."""
+    b, gb = gpu_run(ctx, model, src, rec, steps, stride, tiles=(4, 8, 0))
+    assert np.array_equal(gb, gather)
+    for f in range(9):
+        assert np.array_equal(b.download_field(f), g.download_field(f)), f
+    b.release()
+    log(f"{name}: balanced one-round shape bit-equal to the default shape")
 
 
 @pytest.mark.timeout(900)
@@ -117,6 +131,7 @@ def test_grid_200_parity(gpu):
     g, gather = gpu_run(ctx, model, src, rec, steps, stride)
     assert np.any(gather)
     oracle_parity(ctx, model, src, rec, steps, stride, gather, g, "200^3")
+    same_as_balanced(ctx, model, src, rec, steps, stride, gather, g, "200^3")
     g.release()
 
 
