@@ -201,3 +201,24 @@ def test_stream_equals_direct_bitwise(gpu):
     for f in range(9):
         for k in range(1, len(out)):
             assert np.array_equal(out[0][f], out[k][f]), (f, k)
+
+
+@pytest.mark.parametrize("grid", [(300, 300, 400), (128, 16, 200)], ids=["no-round-fits", "thin-y"])
+def test_balanced_shape_where_one_round_does_not_fit(gpu, grid):
+    """The balanced one-round request (64x8 tiles, z_chunk 0) on a grid whose columns do not fit
+    one round falls back to uniform chunks, and on a grid with few columns caps its chunks at 8
+    planes; either way it steps the same bits as the default shape."""
+    c = W.imported_context(*grid, max_vp=4500.0, pml=20)
+    model = W.layered_model(c.nx, c.ny, c.nz, seed=4)
+    src = W.config2_source(c)
+    out = []
+    for tiles in [(4, 4, 16), (4, 8, 0)]:
+        g = S.GPUSimData3D(c)
+        g.set_tiles(*tiles)
+        g.upload_model(*model)
+        g.run(src, 0, 24)
+        out.append([g.download_field(f) for f in range(9)])
+        g.release()
+    assert np.any(out[0][2])
+    for f in range(9):
+        assert np.array_equal(out[0][f], out[1][f]), f
