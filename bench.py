@@ -661,7 +661,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{dominant[0]}_{FAMILY_KERNEL.get(family, 'kernel')}",
+            "kernel": f"{dominant[0]}_{FAMILY_KERNEL.get(family, 'kernel')}"
+                      + ("_balanced" if family == 4 and z_chunk == 0 else ""),
             "achieved": round(dominant[3], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
