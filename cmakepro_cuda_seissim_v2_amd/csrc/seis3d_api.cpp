@@ -1303,8 +1303,9 @@ bool multi_shot_ok(const std::vector<seis3d_t*>& sessions) {
     return true;
 }
 
-// The LDS-DMA shape a shot steps with in a multi-shot launch: its own when it uses that family;
-// the default shape for a shot on the direct family (the two families round identically).
+// The LDS-DMA shape a shot steps with in a multi-shot launch: its own when it uses that family
+// with uniform chunks; the default shape for a shot on the direct family or on the balanced
+// one-round launch (z_chunk 0), which is a single-shot launch (every shape rounds identically).
 seis3d::KernelConfig multi_shot_shape(const seis3d_t* t) {
     seis3d::KernelConfig k = t->kernels;
     if (k.family != 4 || (k.tile_y != 4 && k.tile_y != 8) || k.z_chunk < 1 || k.z_chunk > 128) {
