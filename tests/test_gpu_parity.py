@@ -222,3 +222,30 @@ def test_balanced_shape_where_one_round_does_not_fit(gpu, grid):
     assert np.any(out[0][2])
     for f in range(9):
         assert np.array_equal(out[0][f], out[1][f]), f
+
+
+def test_balanced_shape_random_grids(gpu):
+    """Seeded grids of assorted sizes, CPML widths and boundaries (all-CPML, free top, reflecting
+    walls): the balanced one-round shape steps the same bits as the default shape."""
+    rng = np.random.default_rng(17)
+    for case in range(10):
+        nx, ny, nz = (int(v) for v in rng.integers(16, 140, 3))
+        pml = int(rng.integers(8, 25))
+        mode = case % 3
+        c = W.imported_context(nx, ny, nz, max_vp=4500.0, pml=pml, free_surface=mode == 1,
+                               reflecting=int(mode == 2))
+        model = W.layered_model(c.nx, c.ny, c.nz, seed=case)
+        src = S.default_source(x=nx // 2, y=ny // 2, z=nz // 3, kind=S.EXPLOSION if case % 2 else S.FORCE,
+                               amplitude=1e8, direction=(0.3, 0.4, 0.866))
+        out = []
+        for tiles in [(4, 4, 16), (4, 8, 0)]:
+            g = S.GPUSimData3D(c)
+            g.set_tiles(*tiles)
+            g.upload_model(*model)
+            g.run(src, 0, 40)
+            out.append([g.download_field(f) for f in range(9)])
+            g.release()
+        print(f"case {case}: {nx}x{ny}x{nz} pml {pml} mode {mode}", flush=True)
+        for f in range(9):
+            assert np.array_equal(out[0][f], out[1][f]), (case, f)
+        assert np.any(out[0][3])
