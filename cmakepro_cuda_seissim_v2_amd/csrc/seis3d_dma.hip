@@ -267,6 +267,10 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const Step& P, 
         const VelTiles<BY>& s = ring[(k - k0) % 3];
         wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
+        // The wave issues this iteration's loads at raised priority (s_setprio: SIMD issue
+        // arbitration among resident waves; 2 here, 3 for the ring copies below), so that the
+        // memory pipeline is fed before the other waves' arithmetic.
+        __builtin_amdgcn_s_setprio(2);
         if (k + 1 <= last) load_operands(k + 1, nxt);
         // x/y CPML memories of plane k, used after the update below has run.
         float px[3], py[3];
@@ -280,6 +284,7 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const Step& P, 
                 for (int c = 0; c < 3; ++c) py[c] = gptr<TABLE>(P.p.psi[1])[psi_index_y(g, c, qy, x, k)];
             }
         }
+        __builtin_amdgcn_s_setprio(0);
 
         shift(q_xz, s.xz[ty * 72 + tx + 4]);
         shift(q_yz, s.yz[(ty + 4) * 64 + tx]);
@@ -406,7 +411,9 @@ __device__ __forceinline__ void velocity_tile(VelShared<BY>& sh, const Step& P, 
         // Last vector-memory instruction of the iteration: plane k+2 into the
         // buffer every wave finished reading before this iteration's barrier
         // (past the chunk end a clamped plane fills a buffer nobody reads).
+        __builtin_amdgcn_s_setprio(3);
         copy_plane(k + 2, min(k + 2, last));
+        __builtin_amdgcn_s_setprio(0);
     };
     int k = k0;
     for (; k + 1 <= last; k += 2) {
@@ -532,6 +539,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const Step& P, co
         const StrTiles<BY>& s = ring[(k - k0) % 3];
         wait_for_ring<NDMA, NLITE>(full_plane(k + 1));
         __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_s_setprio(2);   // loads at raised issue priority, as in velocity_tile
         if (k + 1 <= last) load_operands(k + 1, nxt);
         // x/y CPML memories (slots 3-5) of plane k, used after the update below has run.
         float px[3], py[3];
@@ -545,6 +553,7 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const Step& P, co
                 for (int j = 0; j < 3; ++j) py[j] = gptr<TABLE>(P.p.psi[1])[psi_index_y(g, 3 + j, qy, x, k)];
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         const int c = (ty + 4) * 72 + tx + 4;
         shift(q_vx, s.vx[c]);
         shift(q_vy, s.vy[c]);
@@ -687,7 +696,9 @@ __device__ __forceinline__ void stress_tile(StrShared<BY>& sh, const Step& P, co
             }
         }
         shift(p00, d00); shift(p11, d11); shift(p01, s01); shift(p20, d20); shift(p21, d21);
+        __builtin_amdgcn_s_setprio(3);
         copy_plane(k + 2, min(k + 2, last));
+        __builtin_amdgcn_s_setprio(0);
     };
     int k = k0;
     for (; k + 1 <= last; k += 2) {
