@@ -291,7 +291,10 @@ int seis3d_step_bytes(const seis3d_ctx* ctx, uint64_t* total, uint64_t* velocity
 
 /* Time each candidate kernel shape for `steps` steps on a scratch session of
  * the same grid, keep the fastest for this session and cache it per grid
- * (padded dims, CPML, boundaries) for the process. */
+ * (padded dims, CPML, boundaries) for the process.  Candidates: the direct
+ * family; 64x4 and 64x8 tiles with z-chunks of 8-32 planes and the lengths that
+ * fill whole rounds of resident workgroups; and the balanced one-round launch
+ * (tile_y 8, z_chunk 0; see seis3d_set_tiles). */
 int seis3d_autotune(seis3d_t* t, const seis3d_source* src, int steps, int* family, int* tile_y,
                     int* z_chunk, double* best_ms_per_step);
 
